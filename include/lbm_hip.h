@@ -1,0 +1,154 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X lattice-Boltzmann step library
+ * (liblbm_hip.so, HIP kernels for gfx950).
+ *
+ * This ABI replaces the per-cell passes that SFCMM/LBM's LBMSolver::timeStep
+ * runs on the CPU (src/lbm/solver.cpp:307-320) together with the boundary
+ * application of LBMBndManager (src/lbm/bnd/bnd.h:48-65).  Plain types only:
+ * no C++ or torch types cross it, every entry point returns an int status
+ * (LBM_OK = 0, negative on error) and never throws.  A handle owns its device
+ * buffers and one HIP stream; it is not thread-safe.
+ *
+ * Cell order on this ABI is structured: index = x + nx*(y + ny*z).  The host
+ * layer (lbm_amd/host) maps the reference's Hilbert-ordered cell ids onto it.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LBM_OK 0
+#define LBM_EINVAL (-1)   /* bad argument */
+#define LBM_EHIP (-2)     /* HIP runtime error */
+#define LBM_ENOMEM (-3)   /* device allocation failed */
+#define LBM_DIVERGED (-4) /* NaN/Inf found by a checked step */
+
+/* lattice: the model string of the reference (src/lbm/constants.h:155-157) */
+#define LBM_D2Q9 9
+#define LBM_D3Q19 19
+
+/* population storage.  Every form stores f_i - w_i ("shifted"); collision
+ * runs in FP32 for F32/F16/BF16 and in FP64 for F64. */
+#define LBM_STORE_F32 0
+#define LBM_STORE_F64 1
+#define LBM_STORE_F16 2
+#define LBM_STORE_BF16 3
+
+/* face kinds for the six faces -x,+x,-y,+y,-z,+z (direction ids of the
+ * reference, include/common/constants.h:104-125) */
+#define LBM_FACE_PERIODIC 0    /* 'periodic', generateBndry=false (src/cartesiangrid.h:645-706) */
+#define LBM_FACE_WALL 1        /* 'wall'/'bounceback' (+tangentialVelocity), 'dirichlet'/'bounceback'
+                                  (src/lbm/bnd/bnd_dirichlet.h:79-121) */
+#define LBM_FACE_PRESSURE_ABB 2 /* 'pressure' anti-bounce-back (src/lbm/bnd/bnd_pressure.h:55-106) */
+#define LBM_FACE_VELOCITY_ZH 3 /* Zou-He velocity inlet (no reference counterpart) */
+#define LBM_FACE_PRESSURE_ZH 4 /* Zou-He pressure outlet (no reference counterpart) */
+
+/* cell flags (optional byte per cell) */
+#define LBM_CELL_FLUID 0
+#define LBM_CELL_SOLID 1 /* bounce-back obstacle; its populations are never read or written */
+
+typedef struct lbm_handle lbm_handle;
+
+/* Structured lattice.  Replaces the grid handed over by
+ * LBMSolver::transferGrid (src/lbm/solver.cpp:759-801) for single-level
+ * grids.  solid: optional nx*ny*nz bytes (LBM_CELL_*), copied. */
+typedef struct {
+  int32_t dim;         /* 2 or 3 */
+  int32_t nx, ny, nz;  /* nz = 1 in 2D */
+  const uint8_t *solid;
+} lbm_grid_desc;
+
+/* Solver parameters.  omega = 1/tau as set from 'relaxation'
+ * (src/lbm/solver.cpp:108-112).  force is a constant Guo body force (new). */
+typedef struct {
+  int32_t lattice; /* LBM_D2Q9 or LBM_D3Q19 */
+  int32_t storage; /* LBM_STORE_* */
+  double omega;
+  double force[3];
+} lbm_params;
+
+/* One boundary face.  Replaces the per-surface LBMBndInterface objects that
+ * LBMBndManager::setupBndryCnds builds (src/lbm/bnd/bnd.h:71-142).
+ * value: wall/inlet velocity; rho: pressure (density) for pressure faces. */
+typedef struct {
+  int32_t face; /* 0..5 = -x,+x,-y,+y,-z,+z */
+  int32_t kind; /* LBM_FACE_* */
+  double value[3];
+  double rho;
+} lbm_bc_desc;
+
+/* Reductions in FP64, summed in a fixed order (deterministic).
+ * abs_diff follows the reference's variable order u,v,(w),rho and is its
+ * convergence measure sum|q^n - q^(n-1)| (src/lbm/solver.cpp:233-263,808-815);
+ * res_rel = sqrt(diff_sq / norm_sq) is the relative L2 residual. */
+typedef struct {
+  double mass;
+  double momentum[3];
+  double abs_diff[4];
+  double diff_sq;
+  double norm_sq;
+  double res_rel;
+  double nonfinite;
+  double fluid_cells;
+} lbm_sums;
+
+/* Create a solver.  Allocates two population buffers (A-B pull scheme). */
+int lbm_create(const lbm_grid_desc *grid, const lbm_params *params, lbm_handle **out);
+
+/* Set boundary faces; every non-periodic face must be described once.
+ * Faces not listed are periodic. */
+int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n);
+
+/* Initial condition of src/lbm/solver.cpp:267-295: f = feq(rho, u), then the
+ * first collision.  rho[n], u[n*dim] (cell-major, structured order). */
+int lbm_init_fields(lbm_handle *h, const double *rho, const double *u);
+
+/* Advance nsteps time steps (LBMSolver::timeStep, src/lbm/solver.cpp:307-320). */
+int lbm_step(lbm_handle *h, int32_t nsteps);
+
+/* Advance two steps; the second reports the one-step change of the
+ * macroscopic fields plus mass and momentum (convergenceCondition,
+ * src/lbm/solver.cpp:233-263).  Returns LBM_DIVERGED if NaN/Inf appear. */
+int lbm_step_check(lbm_handle *h, lbm_sums *out);
+
+/* Macroscopic fields after the last step (updateMacroscopicValues,
+ * src/lbm/solver.cpp:513-553): rho[n], u[n*dim], cell-major. */
+int lbm_get_macros(lbm_handle *h, double *rho, double *u);
+
+/* Mass and momentum of the current state (diff fields are zero). */
+int lbm_reduce(lbm_handle *h, lbm_sums *out);
+
+/* Post-collision populations f_i (unshifted), SoA [Q][n], FP64. */
+int lbm_get_populations(lbm_handle *h, double *f);
+int lbm_set_populations(lbm_handle *h, const double *f);
+
+/* Time nsteps steps on the handle's stream with HIP events (ms total). */
+int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms);
+
+/* Launch shape of the step kernel: block_x threads along x (multiple of 64),
+ * block_y rows; nontemporal stores on/off.  0 keeps the current value. */
+int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nontemporal);
+
+/* Size facts: number of cells, populations per cell, bytes per stored
+ * population, and the name of the step kernel symbol family. */
+int lbm_info(const lbm_handle *h, int64_t *ncell, int32_t *q, int32_t *bytes_per_pop);
+
+int lbm_synchronize(lbm_handle *h);
+int lbm_destroy(lbm_handle *h);
+
+/* Last error message of a handle, or of the calling thread when h is NULL. */
+const char *lbm_last_error(const lbm_handle *h);
+
+/* Device copy kernel over n bytes (the bandwidth ceiling used to calibrate
+ * the roofline counters); ms = average time per launch over reps. */
+int lbm_copy_bench(int64_t nbytes, int32_t reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_HIP_H */

@@ -1,0 +1,322 @@
+"""Python side of the FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, and only as the checker.  It wraps
+``liblbm_oracle.so`` (lbm_oracle.c) and builds, for a structured box, the
+reference-style description the oracle steps: a downstream neighbour table
+``nbr[c][i]`` (src/cartesiangrid.h:111-124, -1 = missing) and per-cell
+boundary records applied after propagation in list order
+(src/lbm/bnd/bnd.h:60-65).
+
+Face rule for a direction whose downstream neighbour leaves the box: the
+crossed face decides the boundary; when two faces are crossed a wall wins,
+else the lower axis.  Missing neighbours inside the box (solid cells) are
+plain no-slip bounce-back.  Zou-He records come after the per-direction ones.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liblbm_oracle.so")
+REF_HILBERT = os.path.join(HERE, "_ref", "libref_hilbert.so")
+
+FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4}
+
+
+class OrcBC(ct.Structure):
+    _fields_ = [("cell", ct.c_int64), ("kind", ct.c_int32), ("dirmask", ct.c_int32), ("axis", ct.c_int32),
+                ("sign", ct.c_int32), ("n1", ct.c_int64), ("n2", ct.c_int64), ("u", ct.c_double * 3),
+                ("rho", ct.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run make -C oracle")
+        L = ct.CDLL(LIB)
+        dp = ct.POINTER(ct.c_double)
+        i64p = ct.POINTER(ct.c_int64)
+        L.orc_tables.argtypes = [ct.c_int, dp, dp, ct.POINTER(ct.c_int)]
+        L.orc_feq.argtypes = [ct.c_int, ct.c_double, dp, dp]
+        L.orc_macros.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
+        L.orc_step.argtypes = [ct.c_int, ct.c_int64, i64p, ct.c_double, dp, dp, dp, dp, ct.c_int64,
+                               ct.POINTER(OrcBC)]
+        L.orc_init.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
+        L.orc_residual.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
+        L.orc_hilbert_index.argtypes = [ct.c_int, dp, ct.c_int64]
+        L.orc_hilbert_index.restype = ct.c_int64
+        L.orc_round_bf16.argtypes = [ct.c_double]
+        L.orc_round_bf16.restype = ct.c_double
+        L.orc_round_f16.argtypes = [ct.c_double]
+        L.orc_round_f16.restype = ct.c_double
+        L.orc_num_threads.restype = ct.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def tables(Q):
+    c = np.zeros((Q, 3))
+    w = np.zeros(Q)
+    opp = np.zeros(Q, dtype=np.int32)
+    assert lib().orc_tables(Q, _dp(c), _dp(w), opp.ctypes.data_as(ct.POINTER(ct.c_int))) == 0
+    return c, w, opp
+
+
+def feq(Q, rho, u):
+    out = np.zeros(Q)
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.float64))
+    assert lib().orc_feq(Q, float(rho), _dp(u), _dp(out)) == 0
+    return out
+
+
+def hilbert_index(x, level):
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    return lib().orc_hilbert_index(len(x), _dp(x), int(level))
+
+
+def ref_hilbert_index(x, level):
+    """The reference's own hilbert::index compiled into oracle/_ref (None when absent)."""
+    if not os.path.exists(REF_HILBERT):
+        return None
+    L = ct.CDLL(REF_HILBERT)
+    L.ref_hilbert_index.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.c_longlong]
+    L.ref_hilbert_index.restype = ct.c_longlong
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    return L.ref_hilbert_index(len(x), _dp(x), int(level))
+
+
+class Problem:
+    """A structured box described the reference's way for the oracle.
+
+    shape: (nx, ny[, nz]); faces: {"-x": (kind, u, rho), ...}, unlisted = periodic;
+    solid: optional bool array of the shape (x fastest when flattened).
+    Cell index c = x + nx*(y + ny*z), the same order the GPU library uses.
+    """
+
+    def __init__(self, shape, faces=None, solid=None, omega=1.0, force=(0.0, 0.0, 0.0)):
+        shape = tuple(int(s) for s in shape)
+        if len(shape) == 2:
+            shape = shape + (1,)
+        self.shape = shape
+        nx, ny, nz = shape
+        self.dim = 2 if nz == 1 else 3
+        self.Q = 9 if self.dim == 2 else 19
+        self.omega = float(omega)
+        self.force = np.array(list(force) + [0.0] * (3 - len(force)), dtype=np.float64)[:3]
+        self.faces = {k: (v[0], tuple(v[1]) if v[1] is not None else (0.0, 0.0, 0.0), float(v[2]))
+                      for k, v in (faces or {}).items()}
+        names = ["-x", "+x", "-y", "+y", "-z", "+z"]
+        self.face_kind = [FACE[self.faces[n][0]] if n in self.faces else 0 for n in names]
+        self.face_u = [self.faces[n][1] if n in self.faces else (0.0, 0.0, 0.0) for n in names]
+        self.face_rho = [self.faces[n][2] if n in self.faces else 1.0 for n in names]
+        self.periodic = [self.face_kind[2 * a] == 0 for a in range(3)]
+        self.solid = np.zeros(nx * ny * nz, dtype=bool) if solid is None else np.asarray(solid, bool).reshape(-1)
+        self._build()
+
+    def _build(self):
+        nx, ny, nz = self.shape
+        Q, dim = self.Q, self.dim
+        c, w, opp = tables(Q)
+        self.c, self.w, self.opp = c, w, opp
+        N = nx * ny * nz
+        idx = np.arange(N)
+        x = idx % nx
+        y = (idx // nx) % ny
+        z = idx // (nx * ny)
+        ext = [nx, ny, nz]
+        coords = [x, y, z]
+        nbr = np.full((N, Q - 1), -1, dtype=np.int64)
+        crossed_face = np.full((N, Q - 1), -1, dtype=np.int64)
+        for i in range(Q - 1):
+            tgt = []
+            ok = np.ones(N, dtype=bool)
+            face = np.full(N, -1, dtype=np.int64)
+            for a in range(3):
+                t = coords[a] + int(c[i, a])
+                out = (t < 0) | (t >= ext[a])
+                if self.periodic[a]:
+                    t = t % ext[a]
+                else:
+                    f = 2 * a + (1 if c[i, a] > 0 else 0)
+                    # a wall wins over other faces, else the lower axis
+                    take = out & ((face < 0) | ((self.face_kind[f] == FACE["wall"]) &
+                                                (np.array(self.face_kind)[np.maximum(face, 0)] != FACE["wall"])))
+                    face = np.where(take, f, face)
+                    ok &= ~out
+                    t = np.clip(t, 0, ext[a] - 1)
+                tgt.append(t)
+            n = tgt[0] + nx * (tgt[1] + ny * tgt[2])
+            ok &= ~self.solid[n]
+            nbr[:, i] = np.where(ok, n, -1)
+            crossed_face[:, i] = np.where(ok, -1, face)
+        # The oracle's cell list holds fluid cells only (the reference deletes
+        # cells inside bodies); solid cells keep their slots here but are
+        # never anybody's neighbour and get no records.
+        self.fluid = ~self.solid
+        self.nbr = nbr
+        recs = []
+        zh = []
+        for cidx in np.nonzero(self.fluid & (nbr < 0).any(axis=1))[0]:
+            groups = {}
+            for i in range(Q - 1):
+                if nbr[cidx, i] >= 0:
+                    continue
+                f = int(crossed_face[cidx, i])
+                key = f if f >= 0 else "solid"
+                groups.setdefault(key, 0)
+                groups[key] |= 1 << i
+            for key, mask in groups.items():
+                r = OrcBC()
+                r.cell = int(cidx)
+                r.dirmask = mask
+                if key == "solid":
+                    r.kind = 1
+                    recs.append(r)
+                    continue
+                kind = self.face_kind[key]
+                ax, side = key // 2, key % 2
+                if kind == FACE["wall"]:
+                    r.kind = 1
+                    r.u = (ct.c_double * 3)(*self.face_u[key])
+                    recs.append(r)
+                elif kind == FACE["pressure_abb"]:
+                    r.kind = 2
+                    r.rho = self.face_rho[key]
+                    step = -1 if side == 1 else 1  # inward along the axis
+                    co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+                    c1 = list(co); c1[ax] += step
+                    c2 = list(co); c2[ax] += 2 * step
+                    r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
+                    r.n2 = c2[0] + nx * (c2[1] + ny * c2[2])
+                    recs.append(r)
+            # Zou-He: first Zou-He face the cell lies on
+            co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+            for f in range(2 * dim):
+                k = self.face_kind[f]
+                if k not in (FACE["velocity_zh"], FACE["pressure_zh"]):
+                    continue
+                ax, side = f // 2, f % 2
+                if co[ax] != (ext[ax] - 1 if side else 0):
+                    continue
+                r = OrcBC()
+                r.cell = int(cidx)
+                r.kind = 3 if k == FACE["velocity_zh"] else 4
+                r.axis = ax
+                r.sign = 1 if side else -1
+                r.u = (ct.c_double * 3)(*self.face_u[f])
+                r.rho = self.face_rho[f]
+                zh.append(r)
+                break
+        allr = recs + zh
+        self.nrec = len(allr)
+        self.recs = (OrcBC * max(1, len(allr)))(*allr)
+
+    # ------------------------------------------------------------ state
+    def init(self, rho, u):
+        N = int(np.prod(self.shape))
+        rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(N))
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(N, self.dim))
+        self.fold = np.zeros(N * self.Q)
+        self.fpost = np.zeros(N * self.Q)
+        self.vars = np.zeros(N * (self.dim + 1))
+        assert lib().orc_init(self.Q, N, _dp(rho), _dp(u), _dp(self.fold)) == 0
+        self.fold.reshape(N, self.Q)[self.solid] = 0.0
+        self.steps = 0
+
+    def step(self, n=1):
+        N = int(np.prod(self.shape))
+        L = lib()
+        nbr = self.nbr
+        for _ in range(int(n)):
+            rc = L.orc_step(self.Q, N, nbr.ctypes.data_as(ct.POINTER(ct.c_int64)), self.omega, _dp(self.force),
+                            _dp(self.fold), _dp(self.fpost), _dp(self.vars), self.nrec, self.recs)
+            assert rc == 0, rc
+            self.steps += 1
+
+    def macros(self):
+        """(rho[N], u[N,dim]) of fold; solid cells zeroed."""
+        N = int(np.prod(self.shape))
+        v = np.zeros(N * (self.dim + 1))
+        assert lib().orc_macros(self.Q, N, _dp(self.fold), _dp(self.force), _dp(v)) == 0
+        v = v.reshape(N, self.dim + 1)
+        v[self.solid] = 0.0
+        return v[:, self.dim].copy(), v[:, :self.dim].copy()
+
+    def post_collision(self):
+        """f_post of the last step, SoA [Q][N] (the GPU state after steps-1 launches)."""
+        N = int(np.prod(self.shape))
+        return self.fpost.reshape(N, self.Q).T.copy()
+
+
+def residual(dim, vars_new, vars_old):
+    out = np.zeros(dim + 3)
+    vn = np.ascontiguousarray(vars_new, dtype=np.float64)
+    vo = np.ascontiguousarray(vars_old, dtype=np.float64)
+    n = vn.size // (dim + 1)
+    assert lib().orc_residual(dim, n, _dp(vn), _dp(vo), _dp(out)) == 0
+    return out
+
+
+def round_bf16(x):
+    return lib().orc_round_bf16(float(x))
+
+
+def round_f16(x):
+    return lib().orc_round_f16(float(x))
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def taylor_green_noise(shape, u0, sigma, seed):
+    """Taylor-Green vortex at cell centres x=(i+1/2) with k = 2 pi / n per axis,
+    u = u0 (sin kx cos ky cos kz, -cos kx sin ky cos kz, 0), plus Gaussian
+    noise of std sigma from splitmix64 + Box-Muller (seed recorded), so that
+    CPU and GPU start from identical fields.  Returns (rho, u[N,3])."""
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    i = np.arange(N)
+    X = (i % nx + 0.5) * 2 * np.pi / nx
+    Y = ((i // nx) % ny + 0.5) * 2 * np.pi / ny
+    Z = (i // (nx * ny) + 0.5) * 2 * np.pi / nz
+    u = np.zeros((N, 3))
+    u[:, 0] = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    u[:, 1] = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    if sigma > 0:
+        u += sigma * gaussian_splitmix(3 * N, seed).reshape(N, 3)
+    return np.ones(N), u
+
+
+def splitmix64(n, seed):
+    """n uint64 outputs of splitmix64 (Steele, Lea & Flood 2014) from seed."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        s = (np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) & M
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def gaussian_splitmix(n, seed):
+    m = (n + 1) // 2
+    r = splitmix64(2 * m, seed)
+    u1 = ((r[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+    u2 = (r[1::2] >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    rad = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(2 * m)
+    out[0::2] = rad * np.cos(2 * np.pi * u2)
+    out[1::2] = rad * np.sin(2 * np.pi * u2)
+    return out[:n]

@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bars: FP32 storage <= 1e-5 relative L2 on u and rho; FP64 storage <= 1e-11
+(same arithmetic, different summation/FMA order); reduced 16-bit storage
+reports its error against the FP64 oracle and must stay below the stated
+bound.  Sizes are small enough for the oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import capi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+WALLS2 = {"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0)}
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def run_pair(shape, steps, storage, omega, faces=None, solid=None, force=(0.0, 0.0, 0.0), init=None, seed=0):
+    shape3 = tuple(shape) + ((1,) if len(shape) == 2 else ())
+    N = int(np.prod(shape3))
+    dim = 2 if len(shape) == 2 else 3
+    if init is None:
+        rng = np.random.default_rng(seed)
+        rho0 = 1 + 0.01 * rng.standard_normal(N)
+        u0 = 0.02 * rng.standard_normal((N, dim))
+    else:
+        rho0, u0 = init
+    print(f"case shape={shape} steps={steps} storage={storage} faces={list((faces or {}).keys())}", flush=True)
+    lat = capi.Lattice(shape, storage=storage, omega=omega, force=force, faces=faces, solid=solid)
+    lat.init(rho0, u0)
+    lat.step(steps)
+    rho, u = lat.macros()
+    ref = O.Problem(shape, faces=faces, solid=solid, omega=omega, force=force)
+    ref.init(rho0, u0)
+    ref.step(steps)
+    r_rho, r_u = ref.macros()
+    fl = ~ref.solid
+    lat.close()
+    return rel(u[fl], r_u[fl]), rel(rho[fl], r_rho[fl]), (rho, u, r_rho, r_u)
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d3q19_periodic_tgv(storage, tol):
+    shape = (32, 32, 32)
+    init = O.taylor_green_noise(shape, 0.05, 1e-3, seed=1234)
+    eu, er, _ = run_pair(shape, 100, storage, 1 / 0.6, init=init)
+    print(f"d3q19 periodic {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d3q19_non_multiple_of_64(storage, tol):
+    eu, er, _ = run_pair((70, 9, 5), 30, storage, 1 / 0.8)
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d2q9_force_channel(storage, tol):
+    eu, er, _ = run_pair((64, 32), 200, storage, 1 / 0.8, faces=WALLS2, force=(1e-5, 0.0, 0.0))
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d2q9_couette_moving_wall(storage, tol):
+    faces = {"-y": ("wall", None, 1.0), "+y": ("wall", (0.02, 0.0), 1.0)}
+    eu, er, _ = run_pair((16, 24), 300, storage, 1.0, faces=faces)
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d2q9_pressure_abb_channel(storage, tol):
+    faces = dict(WALLS2)
+    faces["-x"] = ("pressure_abb", None, 1.001)
+    faces["+x"] = ("pressure_abb", None, 1.0)
+    eu, er, _ = run_pair((40, 16), 200, storage, 1 / 0.7, faces=faces)
+    assert eu < tol and er < tol
+
+
+def sphere_solid(shape, c, r):
+    nx, ny, nz = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return (((x + 0.5 - c[0]) ** 2 + (y + 0.5 - c[1]) ** 2 + (z + 0.5 - c[2]) ** 2) < r * r).reshape(-1)
+
+
+def channel_faces3(uin=0.04, pout=1.0):
+    f = {k: ("wall", None, 1.0) for k in ["-y", "+y", "-z", "+z"]}
+    f["-x"] = ("velocity_zh", (uin, 0.0, 0.0), 1.0)
+    f["+x"] = ("pressure_zh", None, pout)
+    return f
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d3q19_sphere_channel(storage, tol):
+    shape = (48, 20, 20)
+    solid = sphere_solid(shape, (16, 10, 10), 4.0)
+    N = int(np.prod(shape))
+    init = (np.ones(N), np.zeros((N, 3)))
+    eu, er, _ = run_pair(shape, 150, storage, 1 / 0.56, faces=channel_faces3(), solid=solid, init=init)
+    print(f"sphere channel {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d3q19_moving_lid_and_abb(storage, tol):
+    f = {k: ("wall", None, 1.0) for k in ["-x", "+x", "-y"]}
+    f["+y"] = ("wall", (0.03, 0.0, 0.01), 1.0)
+    f["-z"] = ("pressure_abb", None, 1.0005)
+    f["+z"] = ("pressure_abb", None, 1.0)
+    eu, er, _ = run_pair((12, 10, 14), 80, storage, 1 / 0.65, faces=f)
+    assert eu < tol and er < tol
+
+
+def test_populations_f64_bitwise_close():
+    """Post-collision state after n launches equals the oracle's f_post of step n+1."""
+    shape = (20, 12)
+    rng = np.random.default_rng(5)
+    N = 240
+    rho0 = 1 + 0.01 * rng.standard_normal(N)
+    u0 = 0.02 * rng.standard_normal((N, 2))
+    faces = {"-y": ("wall", (0.01, 0.0), 1.0), "+y": ("wall", None, 1.0)}
+    lat = capi.Lattice(shape, storage="f64", omega=1.3, faces=faces, force=(1e-5, 2e-6, 0))
+    lat.init(rho0, u0)
+    lat.step(7)
+    f = lat.populations()
+    ref = O.Problem(shape, faces=faces, omega=1.3, force=(1e-5, 2e-6, 0))
+    ref.init(rho0, u0)
+    ref.step(8)
+    assert np.abs(f - ref.post_collision()).max() < 1e-14
+    lat.close()
+
+
+def test_step_check_matches_oracle_residual_f64():
+    shape = (24, 16)
+    rng = np.random.default_rng(8)
+    N = 24 * 16
+    rho0 = 1 + 0.01 * rng.standard_normal(N)
+    u0 = 0.02 * rng.standard_normal((N, 2))
+    lat = capi.Lattice(shape, storage="f64", omega=1.1, faces=WALLS2)
+    lat.init(rho0, u0)
+    lat.step(5)
+    s, rc = lat.step_check()  # steps 6 and 7: change fold^7 - fold^6
+    assert rc == 0
+    ref = O.Problem(shape, faces=WALLS2, omega=1.1)
+    ref.init(rho0, u0)
+    ref.step(6)
+    r6, u6 = ref.macros()
+    ref.step(1)
+    r7, u7 = ref.macros()
+    v6 = np.concatenate([u6, r6[:, None]], 1).reshape(-1)
+    v7 = np.concatenate([u7, r7[:, None]], 1).reshape(-1)
+    res = O.residual(2, v7, v6)
+    assert np.allclose(s.abs_diff[:3], res[:3], rtol=1e-9)
+    assert abs(s.diff_sq - res[3]) < 1e-9 * res[3]
+    assert abs(s.norm_sq - res[4]) < 1e-9 * res[4]
+    assert abs(s.mass - r7.sum()) < 1e-10
+    assert np.allclose(s.momentum[:2], (r7[:, None] * u7).sum(0), atol=1e-12)
+    lat.close()
+
+
+def test_step_check_deterministic():
+    shape = (64, 48, 40)
+    init = O.taylor_green_noise(shape, 0.05, 1e-3, seed=3)
+    out = []
+    for _ in range(2):
+        lat = capi.Lattice(shape, storage="f32", omega=1 / 0.6)
+        lat.init(*init)
+        lat.step(3)
+        s, rc = lat.step_check()
+        out.append(s.as_dict())
+        lat.close()
+    assert out[0] == out[1]
+
+
+def test_divergence_reported():
+    lat = capi.Lattice((16, 8, 8), storage="f32", omega=1.0)
+    N = 16 * 8 * 8
+    lat.init(np.ones(N), np.zeros((N, 3)))
+    f = lat.populations()
+    f[3, 17] = np.nan
+    lat.set_populations(f)
+    s, rc = lat.step_check()
+    assert rc == capi.LBM_DIVERGED and s.nonfinite >= 1
+    lat.close()
+
+
+def test_invalid_arguments_rejected():
+    with pytest.raises(capi.LBMError):
+        capi.Lattice((2, 8, 8), storage="f32", omega=1.0)
+    with pytest.raises(capi.LBMError):
+        capi.Lattice((8, 8, 8), storage="f32", omega=2.5)
+    lat = capi.Lattice((8, 8, 8), storage="f32", omega=1.0)
+    with pytest.raises(capi.LBMError):
+        lat.set_faces({"-x": ("wall", None, 1.0)})  # periodic partner missing
+    with pytest.raises(capi.LBMError):
+        lat.set_launch(96, 2, 0)
+    lat.close()
+
+
+@pytest.mark.parametrize("storage,bound", [("f16", 2e-3), ("bf16", 3e-2)])
+def test_reduced_storage_error_stated(storage, bound):
+    shape = (32, 32, 32)
+    init = O.taylor_green_noise(shape, 0.05, 1e-3, seed=1234)
+    eu, er, _ = run_pair(shape, 100, storage, 1 / 0.6, init=init)
+    print(f"d3q19 shifted {storage}: rel-L2 vs FP64 oracle u={eu:.3e} rho={er:.3e}")
+    assert eu < bound and er < bound
+
+
+@pytest.mark.parametrize("launch", [(64, 4, 0), (128, 2, 1), (256, 1, 1)])
+def test_launch_shapes_identical(launch):
+    shape = (128, 16, 8)
+    init = O.taylor_green_noise(shape, 0.05, 1e-3, seed=2)
+    res = []
+    for lz in [(256, 1, 0), launch]:
+        lat = capi.Lattice(shape, storage="f32", omega=1 / 0.6)
+        lat.set_launch(*lz)
+        lat.init(*init)
+        lat.step(10)
+        res.append(lat.populations())
+        lat.close()
+    assert np.array_equal(res[0], res[1])

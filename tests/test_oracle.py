@@ -1,0 +1,281 @@
+"""Pins the FP64 oracle (oracle/lbm_oracle.c) before anything is compared to it.
+
+Each check is chosen so that a plausible slip (a wrong table entry, a sign,
+a transposed index, a dropped term) fails: reference fixtures
+(tests/golden), the reference's own Hilbert index built into oracle/_ref,
+closed-form moments, exact streaming permutations, conservation laws and
+analytic flows.
+"""
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.mark.parametrize("Q,name", [(9, "D2Q9"), (19, "D3Q19")])
+def test_tables_match_reference_fixture(Q, name):
+    ref = json.load(open(os.path.join(GOLD, "lattice_tables.json")))["tables"][name]
+    c, w, opp = O.tables(Q)
+    dim = 2 if Q == 9 else 3
+    assert np.array_equal(c[:, :dim], np.array(ref["c"]))
+    assert np.allclose(w, [float(Fraction(s)) for s in ref["w"]], rtol=0, atol=1e-17)
+    assert list(opp) == ref["opp"]
+
+
+@pytest.mark.parametrize("Q", [9, 19])
+def test_lattice_invariants(Q):
+    c, w, opp = O.tables(Q)
+    dim = 2 if Q == 9 else 3
+    assert np.all(c[Q - 1] == 0), "rest population must be last"
+    assert abs(w.sum() - 1) < 1e-15
+    assert np.abs(w @ c).max() < 1e-15
+    m2 = np.einsum("i,ia,ib->ab", w, c, c)[:dim, :dim]
+    assert np.allclose(m2, np.eye(dim) / 3, atol=1e-15)
+    m3 = np.einsum("i,ia,ib,ic->abc", w, c, c, c)
+    assert np.abs(m3).max() < 1e-15
+    m4 = np.einsum("i,ia,ib,ic,id->abcd", w, c, c, c, c)[:dim, :dim, :dim, :dim]
+    d = np.eye(dim)
+    iso = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 9
+    assert np.allclose(m4, iso, atol=1e-15)
+    assert np.array_equal(opp[opp], np.arange(Q))
+    assert np.array_equal(c[opp], -c)
+
+
+@pytest.mark.parametrize("Q", [9, 19])
+def test_equilibrium_moments(Q):
+    c, w, _ = O.tables(Q)
+    dim = 2 if Q == 9 else 3
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        rho = 0.9 + 0.2 * rng.random()
+        u = 0.1 * (rng.random(dim) - 0.5)
+        f = O.feq(Q, rho, u)
+        assert abs(f.sum() - rho) < 1e-14
+        assert np.allclose(f @ c[:, :dim], rho * u, atol=1e-15)
+        P = np.einsum("i,ia,ib->ab", f, c[:, :dim], c[:, :dim])
+        assert np.allclose(P, rho * np.outer(u, u) + rho / 3 * np.eye(dim), atol=1e-14)
+        # closed form of one population: w rho (1 + 3cu + 4.5cu^2 - 1.5u^2)
+        i = 1
+        cu = c[i, :dim] @ u
+        assert abs(f[i] - w[i] * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * u @ u)) < 1e-15
+
+
+def test_hilbert_known_answers():
+    kat = json.load(open(os.path.join(GOLD, "hilbert_kat.json")))["cases"]
+    assert len(kat) > 100
+    for k in kat:
+        if k["dim"] > 3:
+            continue
+        assert O.hilbert_index(k["x"], k["level"]) == k["index"], k
+
+
+def test_hilbert_against_reference_build():
+    if O.ref_hilbert_index((0.1, 0.2), 1) is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(0)
+    for dim in (2, 3):
+        for _ in range(300):
+            x = rng.random(dim)
+            lvl = int(rng.integers(1, 9))
+            assert O.hilbert_index(x, lvl) == O.ref_hilbert_index(x, lvl)
+
+
+@pytest.mark.parametrize("shape", [(7, 5), (5, 4, 6)])
+def test_streaming_is_exact_shift(shape):
+    """omega = 0 makes collision the identity, so one step moves every
+    population by c_i exactly (periodic box): brute force over all cells."""
+    p = O.Problem(shape, omega=1e-300)
+    N = int(np.prod(p.shape))
+    rng = np.random.default_rng(1)
+    f0 = rng.random(N * p.Q)
+    p.init(np.ones(N), np.zeros((N, p.dim)))
+    p.fold[:] = f0
+    p.omega = 0.0
+    p.step(1)
+    f0 = f0.reshape(N, p.Q)
+    f1 = p.fold.reshape(N, p.Q)
+    nx, ny, nz = p.shape
+    for cidx in range(N):
+        x, y, z = cidx % nx, (cidx // nx) % ny, cidx // (nx * ny)
+        for i in range(p.Q):
+            sx = (x - int(p.c[i, 0])) % nx
+            sy = (y - int(p.c[i, 1])) % ny
+            sz = (z - int(p.c[i, 2])) % nz
+            assert f1[cidx, i] == f0[sx + nx * (sy + ny * sz), i]
+
+
+@pytest.mark.parametrize("shape", [(12, 10), (8, 6, 7)])
+def test_periodic_conservation(shape):
+    p = O.Problem(shape, omega=1 / 0.7)
+    N = int(np.prod(p.shape))
+    rng = np.random.default_rng(2)
+    rho0 = 1 + 0.01 * rng.standard_normal(N)
+    u0 = 0.03 * rng.standard_normal((N, p.dim))
+    p.init(rho0, u0)
+    c = p.c[:, :p.dim]
+    mass0 = p.fold.sum()
+    mom0 = p.fold.reshape(N, p.Q).sum(0) @ c
+    p.step(25)
+    assert abs(p.fold.sum() - mass0) < 1e-11
+    assert np.allclose(p.fold.reshape(N, p.Q).sum(0) @ c, mom0, atol=1e-12)
+
+
+def test_guo_force_adds_exact_momentum():
+    """Periodic box: every step adds F per cell to the total momentum."""
+    F = (1e-5, -2e-5)
+    p = O.Problem((10, 8), omega=1.2, force=F)
+    N = 80
+    p.init(np.ones(N), np.zeros((N, 2)))
+    c = p.c[:, :2]
+    m0 = p.fold.reshape(N, p.Q).sum(0) @ c
+    p.step(7)
+    m1 = p.fold.reshape(N, p.Q).sum(0) @ c
+    assert np.allclose(m1 - m0, 7 * N * np.array(F), rtol=1e-10, atol=1e-16)
+
+
+def test_bounce_back_closed_box_rest_and_mass():
+    faces = {k: ("wall", None, 1.0) for k in ["-x", "+x", "-y", "+y", "-z", "+z"]}
+    p = O.Problem((6, 5, 4), faces=faces, omega=1.1)
+    N = p.fold.size // p.Q if hasattr(p, "fold") else 120
+    N = 6 * 5 * 4
+    rng = np.random.default_rng(5)
+    p.init(1 + 0.01 * rng.standard_normal(N), 0.01 * rng.standard_normal((N, 3)))
+    mass0 = p.fold.sum()
+    p.step(40)
+    assert abs(p.fold.sum() - mass0) < 1e-12
+    # rest state is a fixed point
+    q = O.Problem((6, 5, 4), faces=faces, omega=1.1)
+    q.init(np.ones(N), np.zeros((N, 3)))
+    f0 = q.fold.copy()
+    q.step(10)
+    assert np.abs(q.fold - f0).max() < 1e-15
+
+
+def test_poiseuille_bounce_back_exact_tau():
+    """Force-driven channel with halfway bounce-back: at tau = 1/2 + sqrt(3/16)
+    the steady BGK solution is the exact parabola with walls halfway between
+    nodes (He, Zou, Luo & Dembo 1997, the special case the reference's
+    src/lbm/analytical_solutions.h:41 names)."""
+    H = 12
+    tau = 0.5 + math.sqrt(3.0 / 16.0)
+    nu = (2 * tau - 1) / 6
+    g = 1e-6
+    p = O.Problem((4, H), faces={"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0)}, omega=1 / tau,
+                  force=(g, 0.0))
+    N = 4 * H
+    p.init(np.ones(N), np.zeros((N, 2)))
+    p.step(6000)
+    rho, u = p.macros()
+    y = np.arange(H) + 0.5
+    ua = g / (2 * nu) * y * (H - y)
+    uy = u.reshape(H, 4, 2)[:, 0, 0]
+    assert np.abs(uy - ua).max() / ua.max() < 1e-9
+
+
+def test_moving_wall_couette():
+    """Ladd moving-wall bounce-back (rho0 = 1): steady Couette profile is linear
+    through the halfway walls."""
+    H = 10
+    U = 0.01
+    p = O.Problem((3, H), faces={"-y": ("wall", None, 1.0), "+y": ("wall", (U, 0.0), 1.0)}, omega=1.0)
+    N = 3 * H
+    p.init(np.ones(N), np.zeros((N, 2)))
+    p.step(3000)
+    rho, u = p.macros()
+    y = np.arange(H) + 0.5
+    assert np.abs(u.reshape(H, 3, 2)[:, 1, 0] - U * y / H).max() < 1e-10
+
+
+@pytest.mark.parametrize("shape,face", [((9, 7), "-x"), ((7, 6, 5), "-x"), ((6, 7, 5), "+y")])
+def test_zou_he_velocity_moments(shape, face):
+    """After a step, every cell on a Zou-He velocity face carries exactly the
+    prescribed velocity (the construction's defining property)."""
+    dim = len(shape)
+    uin = (0.03, 0.004, -0.002)[:dim]
+    faces = {"-x": ("wall", None, 1.0), "+x": ("wall", None, 1.0), "-y": ("wall", None, 1.0),
+             "+y": ("wall", None, 1.0)}
+    if dim == 3:
+        faces.update({"-z": ("wall", None, 1.0), "+z": ("wall", None, 1.0)})
+    faces[face] = ("velocity_zh", uin, 1.0)
+    p = O.Problem(shape, faces=faces, omega=1.3)
+    N = int(np.prod(p.shape))
+    rng = np.random.default_rng(9)
+    p.init(1 + 0.01 * rng.standard_normal(N), 0.01 * rng.standard_normal((N, dim)))
+    p.step(3)
+    rho, u = p.macros()
+    nx, ny, nz = p.shape
+    idx = np.arange(N)
+    co = [idx % nx, (idx // nx) % ny, idx // (nx * ny)]
+    ax = "xyz".index(face[1])
+    on = co[ax] == (0 if face[0] == "-" else p.shape[ax] - 1)
+    # interior of the face (edges also touch walls, still prescribed)
+    assert np.abs(u[on] - np.array(uin)).max() < 1e-13
+
+
+def test_zou_he_pressure_density():
+    faces = {"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0), "-x": ("velocity_zh", (0.02, 0.0), 1.0),
+             "+x": ("pressure_zh", None, 0.998)}
+    p = O.Problem((12, 8), faces=faces, omega=1.4)
+    N = 96
+    p.init(np.ones(N), np.zeros((N, 2)))
+    p.step(50)
+    rho, u = p.macros()
+    on = (np.arange(N) % 12) == 11
+    assert np.abs(rho[on] - 0.998).max() < 1e-13
+    assert np.abs(u[on][:, 1]).max() < 1e-13
+
+
+def test_residual_definition():
+    rng = np.random.default_rng(4)
+    a = rng.random((50, 4))
+    b = rng.random((50, 4))
+    r = O.residual(3, a.reshape(-1), b.reshape(-1))
+    assert np.allclose(r[:4], np.abs(a - b).sum(0))
+    assert abs(r[4] - ((a[:, :3] - b[:, :3]) ** 2).sum()) < 1e-12
+    assert abs(r[5] - (a[:, :3] ** 2).sum()) < 1e-12
+
+
+def _shear_wave_error(ny, tau=0.6, t=200):
+    nu = (2 * tau - 1) / 6
+    p = O.Problem((4, ny, 4), omega=1 / tau)
+    N = 4 * ny * 4
+    y = (np.arange(N) // 4) % ny
+    u = np.zeros((N, 3))
+    u[:, 0] = 0.01 * np.sin(2 * np.pi * (y + 0.5) / ny)
+    p.init(np.ones(N), u)
+    p.step(t)
+    _, uu = p.macros()
+    k = 2 * np.pi / ny
+    return math.sqrt((uu ** 2).sum() / (u ** 2).sum()) / math.exp(-nu * k * k * t) - 1
+
+
+def test_shear_wave_viscosity():
+    """A shear wave u_x = u0 sin(k y) is an exact linear Navier-Stokes solution
+    decaying as exp(-nu k^2 t) with nu = (2 tau - 1)/6 (src/lbm/solver.cpp:108-112);
+    the lattice error must be small and shrink at second order."""
+    e32, e64 = _shear_wave_error(32), _shear_wave_error(64)
+    assert abs(e64) < 2e-3
+    assert 3.0 < e32 / e64 < 6.5
+
+
+def test_quantisers():
+    assert O.round_f16(1.0 + 2 ** -11) == 1.0  # tie to even
+    assert O.round_f16(1.0 + 3 * 2 ** -11) == 1.0 + 2 ** -9
+    assert O.round_f16(65520.0) == math.inf
+    assert O.round_f16(2 ** -25) == 0.0 and O.round_f16(3 * 2 ** -26) == 2 ** -24
+    assert O.round_bf16(1.0 + 2 ** -8) == 1.0
+    assert O.round_bf16(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6
+
+
+def test_splitmix_reference_values():
+    # splitmix64 with seed 0: first output 0xe220a8397b1dcdaf (published test vector)
+    assert int(O.splitmix64(1, 0)[0]) == 0xE220A8397B1DCDAF
+    g = O.gaussian_splitmix(200000, 11)
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1) < 0.01
