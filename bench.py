@@ -63,7 +63,17 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="time only the device copy kernel over the step kernel's algorithmic bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from lbm_amd import capi
+
+        nbytes = 2 * 19 * capi.BYTES[args.storage] * args.n ** 3 // 2  # read + write = 2x this
+        ms = capi.copy_bench(nbytes, args.steps)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "ms_per_launch": ms,
+                          "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)

@@ -116,6 +116,18 @@ int lbm_step(lbm_handle *h, int32_t nsteps);
  * src/lbm/solver.cpp:233-263).  Returns LBM_DIVERGED if NaN/Inf appear. */
 int lbm_step_check(lbm_handle *h, lbm_sums *out);
 
+/* One step with an explicit reduction mode:
+ *   0 plain, 1 snapshot (store u, rho), 2 check (one-step change against the
+ *   last snapshot, sums in *out), 3 check and snapshot.
+ * lbm_init_fields stores the snapshot of the initial fields.  This lets a
+ * host reproduce the reference's schedule, where the check at step t
+ * compares the fields of steps t-1 and t-2 (src/lbm/solver.cpp:189-197). */
+#define LBM_STEP_PLAIN 0
+#define LBM_STEP_SNAPSHOT 1
+#define LBM_STEP_CHECK 2
+#define LBM_STEP_CHECK_SNAPSHOT 3
+int lbm_step_ex(lbm_handle *h, int32_t mode, lbm_sums *out);
+
 /* Macroscopic fields after the last step (updateMacroscopicValues,
  * src/lbm/solver.cpp:513-553): rho[n], u[n*dim], cell-major. */
 int lbm_get_macros(lbm_handle *h, double *rho, double *u);

@@ -1,0 +1,56 @@
+/*
+ * lbm_host.h -- C ABI of the drop-in host layer (liblbm_host.so).
+ *
+ * Exposes the grid the host builds from a reference configuration file --
+ * cell ids in the reference's order, centres, neighbour tables, boundary
+ * cells and surfaces -- and a whole solver run, so that callers and tests
+ * can check them without C++ types.  Replaces what the reference's
+ * GridGenerator + CartesianGrid::loadGridInplace produce
+ * (src/gridgenerator/gridGenerator.cpp:72-229, src/cartesiangrid.h:264-320)
+ * and LBMSolver::run (src/lbm/solver.cpp:176-214).
+ */
+#ifndef LBM_HOST_H
+#define LBM_HOST_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_host_grid lbm_host_grid;
+
+/* Generate the grid of a configuration (JSON text) and prepare it for the
+ * solver section (boundary cells, surfaces, periodic links, diagonals). */
+int lbm_host_grid_create(const char *json_text, lbm_host_grid **out);
+int64_t lbm_host_grid_ncells(const lbm_host_grid *g);
+int32_t lbm_host_grid_dim(const lbm_host_grid *g);
+/* neighbour slots per cell: 8 (2D) or 26 (3D) */
+int32_t lbm_host_grid_stride(const lbm_host_grid *g);
+/* centres [n][dim] */
+int lbm_host_grid_centers(const lbm_host_grid *g, double *out);
+/* neighbour table [n][stride], -1 = none */
+int lbm_host_grid_neighbors(const lbm_host_grid *g, int64_t *out);
+/* boundary-cell flag [n] */
+int lbm_host_grid_bndry(const lbm_host_grid *g, uint8_t *out);
+/* structured window extents nxyz[3] and structured index per cell [n] */
+int lbm_host_grid_structured(const lbm_host_grid *g, int64_t *nxyz, int64_t *index);
+/* surfaces in name order: count, then name / cell list / normal dirs of surface i */
+int32_t lbm_host_grid_surface_count(const lbm_host_grid *g);
+int lbm_host_grid_surface(const lbm_host_grid *g, int32_t i, char *name, int32_t name_len, int64_t *ncell,
+                          int64_t *cells, int64_t *dirs);
+void lbm_host_grid_destroy(lbm_host_grid *g);
+
+/* Run the solver on a configuration file like `lbm <file>`; storage may be
+ * NULL (use the file's key).  The final fields are written in reference
+ * cell order into rho[n] and u[n*dim] when those are non-NULL and n matches.
+ * Returns 0, or the process exits with status 255 on a fatal error exactly
+ * like the CLI. */
+int lbm_host_run_file(const char *config_file, const char *storage, int64_t n, double *rho, double *u,
+                      int64_t *steps_done);
+
+const char *lbm_host_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+LIB_PATH = os.environ.get("LBM_HIP_LIB") or os.path.join(HERE, "liblbm_hip.so")
 
 LBM_OK, LBM_EINVAL, LBM_EHIP, LBM_ENOMEM, LBM_DIVERGED = 0, -1, -2, -3, -4
 D2Q9, D3Q19 = 9, 19
@@ -46,7 +46,7 @@ class Sums(ct.Structure):
 
 
 # every symbol declared in include/lbm_hip.h
-EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_get_macros",
+EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_get_macros",
            "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_set_launch",
            "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
@@ -68,6 +68,7 @@ def load():
     lib.lbm_init_fields.argtypes = [P, dp, dp]
     lib.lbm_step.argtypes = [P, ct.c_int32]
     lib.lbm_step_check.argtypes = [P, ct.POINTER(Sums)]
+    lib.lbm_step_ex.argtypes = [P, ct.c_int32, ct.POINTER(Sums)]
     lib.lbm_get_macros.argtypes = [P, dp, dp]
     lib.lbm_reduce.argtypes = [P, ct.POINTER(Sums)]
     lib.lbm_get_populations.argtypes = [P, dp]
@@ -156,6 +157,13 @@ class Lattice:
     def step_check(self):
         s = Sums()
         rc = self.lib.lbm_step_check(self.h, ct.byref(s))
+        if rc not in (LBM_OK, LBM_DIVERGED):
+            self._check(rc)
+        return s, rc
+
+    def step_ex(self, mode):
+        s = Sums()
+        rc = self.lib.lbm_step_ex(self.h, int(mode), ct.byref(s))
         if rc not in (LBM_OK, LBM_DIVERGED):
             self._check(rc)
         return s, rc

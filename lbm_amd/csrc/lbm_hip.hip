@@ -92,27 +92,28 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NSUM], double *pa
   if (tid < NSUM) {
     double s = 0;
     for (int w2 = 0; w2 < nw; ++w2) s += red[w2][tid];
+    // structure of arrays: partials[k * nblocks + block] (coalesced second stage)
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
     const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-    partials[b * NSUM + tid] = s;
+    partials[(long long)tid * nb + b] = s;
   }
 }
 
-// Fixed-order second stage: one block of 256 threads, thread t sums blocks
-// t, t+256, ... in order, then a fixed pairwise tree.
-__global__ void __launch_bounds__(256) k_finalize(const double *partials, long long nblocks, double *out) {
-  __shared__ double s[256];
-  for (int k = 0; k < NSUM; ++k) {
-    double acc = 0;
-    for (long long b = threadIdx.x; b < nblocks; b += 256) acc += partials[b * NSUM + k];
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = s[0];
+// Fixed-order second stage: block k reduces sum k; thread t sums blocks
+// t, t+1024, ... in order, then a fixed pairwise tree.
+__global__ void __launch_bounds__(1024) k_finalize(const double *partials, long long nblocks, double *out) {
+  __shared__ double s[1024];
+  const int k = blockIdx.x;
+  const double *p = partials + (long long)k * nblocks;
+  double acc = 0;
+  for (long long b = threadIdx.x; b < nblocks; b += 1024) acc += p[b];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = s[0];
 }
 
 // --------------------------------------------------------------------------
@@ -251,9 +252,18 @@ __device__ void cell_velocity(const ST *src, long long plane, long long cell, do
 
 // --------------------------------------------------------------------------
 // The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
-// 2: reduce against the snapshot written by the previous step.
+// 2: reduce against the snapshot written by the previous step; 3: both
+// (reduce against snap_in, write snap_out).
+#ifndef LBM_STEP_WAVES_PER_EU
+#define LBM_STEP_WAVES_PER_EU 0
+#endif
+#if LBM_STEP_WAVES_PER_EU > 0
+#define LBM_STEP_ATTR __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
+#else
+#define LBM_STEP_ATTR
+#endif
 template <int Q, typename ST, bool FLAGS, int MODE, bool NT>
-__global__ void __launch_bounds__(256) k_step(const KArgs a) {
+__global__ void __launch_bounds__(256) LBM_STEP_ATTR k_step(const KArgs a) {
   using L = Lat<Q>;
   using C = typename Store<ST>::C;
   const ST *__restrict__ src = static_cast<const ST *>(a.src);
@@ -267,7 +277,7 @@ __global__ void __launch_bounds__(256) k_step(const KArgs a) {
   if (FLAGS && active) active = a.flags[cell] == LBM_CELL_FLUID;
 
   double red[NSUM];
-  if (MODE == 2) {
+  if (MODE >= 2) {
 #pragma unroll
     for (int k = 0; k < NSUM; ++k) red[k] = 0;
   }
@@ -364,12 +374,12 @@ __global__ void __launch_bounds__(256) k_step(const KArgs a) {
 
     const Moments<Q, C> m = moments<Q, C>(h, Fx, Fy, Fz);
     using V4 = typename Vec4<C>::T;
-    if (MODE == 1) {
+    if (MODE == 1 || MODE == 3) {
       V4 v;
       v.x = m.ux; v.y = m.uy; v.z = m.uz; v.w = m.rho;
       static_cast<V4 *>(a.snap_out)[cell] = v;
     }
-    if (MODE == 2) {
+    if (MODE >= 2) {
       const V4 o = static_cast<const V4 *>(a.snap_in)[cell];
       const double ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
       const double dux = ux - (double)o.x, duy = uy - (double)o.y, duz = uz - (double)o.z, drh = rho - (double)o.w;
@@ -388,7 +398,7 @@ __global__ void __launch_bounds__(256) k_step(const KArgs a) {
     }
     collide_store<Q, ST, NT>(h, m, C(a.omega), Fx, Fy, Fz, force, dst, plane, cell);
   }
-  if (MODE == 2) block_reduce_store<256>(red, a.partials);
+  if (MODE >= 2) block_reduce_store<256>(red, a.partials);
 }
 
 // Initial condition: h = feq(rho, u) - w, then the first collision, i.e. the
@@ -420,6 +430,12 @@ __global__ void __launch_bounds__(256) k_init(KArgs a, const double *rho, const 
   const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
   const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
   const Moments<Q, C> m = moments<Q, C>(h, Fx, Fy, Fz);
+  if (a.snap_out) {
+    using V4 = typename Vec4<C>::T;
+    V4 v;
+    v.x = m.ux; v.y = m.uy; v.z = m.uz; v.w = m.rho;
+    static_cast<V4 *>(a.snap_out)[cell] = v;
+  }
   collide_store<Q, ST, false>(h, m, C(a.omega), Fx, Fy, Fz, force, dst, a.plane, cell);
 }
 
@@ -491,9 +507,23 @@ __global__ void k_set_pops(KArgs a, const double *f) {
   for (int j = 0; j < Q; ++j) Store<ST>::st(dst + j * a.plane + cell, C(f[j * n + cc] - L::w[j]));
 }
 
+// Streaming copy used as the bandwidth ceiling: one 16-byte element per
+// thread, nontemporal like the step kernel.
 __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ in, float4 *__restrict__ out, long long n) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    out[i] = in[i];
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float4 *p = in + i;
+    float4 v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    v.z = __builtin_nontemporal_load(&p->z);
+    v.w = __builtin_nontemporal_load(&p->w);
+    float4 *q = out + i;
+    __builtin_nontemporal_store(v.x, &q->x);
+    __builtin_nontemporal_store(v.y, &q->y);
+    __builtin_nontemporal_store(v.z, &q->z);
+    __builtin_nontemporal_store(v.w, &q->w);
+  }
 }
 
 }  // namespace lbm
@@ -513,12 +543,13 @@ struct lbm_handle {
   FaceDev face[6];
   int per[3] = {1, 1, 1};
   double omega = 1.0, F[3] = {0, 0, 0};
-  void *snap = nullptr;
+  void *snap[2] = {nullptr, nullptr};
+  int snap_last = 0;  // buffer holding the most recent snapshot
   double *partials = nullptr;
   long long partials_cap = 0;
   double *sums_d = nullptr;
   hipStream_t stream = nullptr;
-  int bx = 256, by = 1, nt = 0;
+  int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
   long long steps = 0;
   std::string err;
 };
@@ -592,13 +623,15 @@ struct StepL {
         if (nt) hipLaunchKernelGGL((k_step<Q, ST, true, 0, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_step<Q, ST, true, 0, false>), g, b, 0, s, a);
       } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, true, 1, false>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_step<Q, ST, true, 2, false>), g, b, 0, s, a);
+      else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, true, 2, false>), g, b, 0, s, a);
+      else hipLaunchKernelGGL((k_step<Q, ST, true, 3, false>), g, b, 0, s, a);
     } else {
       if (mode == 0) {
         if (nt) hipLaunchKernelGGL((k_step<Q, ST, false, 0, true>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_step<Q, ST, false, 0, false>), g, b, 0, s, a);
       } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, false, 1, false>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_step<Q, ST, false, 2, false>), g, b, 0, s, a);
+      else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, false, 2, false>), g, b, 0, s, a);
+      else hipLaunchKernelGGL((k_step<Q, ST, false, 3, false>), g, b, 0, s, a);
     }
   }
 };
@@ -636,6 +669,8 @@ static int esize_of(int storage) {
   }
   return 0;
 }
+
+static int ensure_snapshots(lbm_handle *h);
 
 static int ensure_partials(lbm_handle *h) {
   const long long nb = nblocks_of(h);
@@ -760,8 +795,12 @@ int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   HIPCHK(h, hipMalloc(&ud, h->ncell * h->dim * sizeof(double)));
   HIPCHK(h, hipMemcpyAsync(rd, rho, h->ncell * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(ud, u, h->ncell * h->dim * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int rc = ensure_snapshots(h);
+  if (rc) return rc;
   KArgs a = make_args(h);
   a.dst = h->buf[h->cur];
+  h->snap_last = 0;
+  a.snap_out = h->snap[0];
   dim3 g, b;
   launch_dims(h, g, b);
   dispatch_qs<InitL>(h->Q, h->storage, a, g, b, h->stream, (const double *)rd, (const double *)ud);
@@ -773,23 +812,42 @@ int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   return LBM_OK;
 }
 
+static void fill_sums(const double *s, int dim, lbm_sums *out);
+
+static int ensure_snapshots(lbm_handle *h) {
+  const size_t cs = h->storage == LBM_STORE_F64 ? sizeof(double4) : sizeof(float4);
+  for (int b = 0; b < 2; ++b)
+    if (!h->snap[b]) HIPCHK(h, hipMalloc(&h->snap[b], (size_t)h->px * h->ny * h->nz * cs));
+  return LBM_OK;
+}
+
 static int launch_step(lbm_handle *h, int mode) {
   KArgs a = make_args(h);
-  if (mode == 1 || mode == 2) {
-    if (!h->snap) {
-      const size_t cs = h->storage == LBM_STORE_F64 ? sizeof(double4) : sizeof(float4);
-      HIPCHK(h, hipMalloc(&h->snap, (size_t)h->px * h->ny * h->nz * cs));
-    }
-    a.snap_in = h->snap;
-    a.snap_out = h->snap;
+  if (mode != 0) {
+    const int rc = ensure_snapshots(h);
+    if (rc) return rc;
+    a.snap_in = h->snap[h->snap_last];
+    a.snap_out = h->snap[h->snap_last ^ 1];
     a.partials = h->partials;
   }
   dim3 g, b;
   launch_dims(h, g, b);
   dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0);
   HIPCHK(h, hipGetLastError());
+  if (mode == 1 || mode == 3) h->snap_last ^= 1;
   h->cur ^= 1;
   h->steps++;
+  return LBM_OK;
+}
+
+static int finalize_sums(lbm_handle *h, lbm_sums *out) {
+  hipLaunchKernelGGL(k_finalize, dim3(NSUM), dim3(1024), 0, h->stream, (const double *)h->partials, nblocks_of(h), h->sums_d);
+  HIPCHK(h, hipGetLastError());
+  double s[NSUM];
+  HIPCHK(h, hipMemcpyAsync(s, h->sums_d, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  fill_sums(s, h->dim, out);
+  if (s[10] > 0) return fail(h, LBM_DIVERGED, "non-finite values (solution diverged)");
   return LBM_OK;
 }
 
@@ -824,13 +882,16 @@ int lbm_step_check(lbm_handle *h, lbm_sums *out) {
   if (rc) return rc;
   rc = launch_step(h, 2);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, h->stream, (const double *)h->partials, nblocks_of(h), h->sums_d);
-  HIPCHK(h, hipGetLastError());
-  double s[NSUM];
-  HIPCHK(h, hipMemcpyAsync(s, h->sums_d, sizeof(s), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  fill_sums(s, h->dim, out);
-  if (s[10] > 0) return fail(h, LBM_DIVERGED, "lbm_step_check: non-finite values (solution diverged)");
+  return finalize_sums(h, out);
+}
+
+int lbm_step_ex(lbm_handle *h, int32_t mode, lbm_sums *out) {
+  if (!h || mode < 0 || mode > 3 || (mode >= 2 && !out)) return fail(h, LBM_EINVAL, "lbm_step_ex: bad argument");
+  int rc = ensure_partials(h);
+  if (rc) return rc;
+  rc = launch_step(h, mode);
+  if (rc) return rc;
+  if (mode >= 2) return finalize_sums(h, out);
   return LBM_OK;
 }
 
@@ -862,7 +923,7 @@ int lbm_reduce(lbm_handle *h, lbm_sums *out) {
   launch_dims(h, g, b);
   dispatch_qs<MacroL>(h->Q, h->storage, a, g, b, h->stream, (double *)nullptr, (double *)nullptr);
   HIPCHK(h, hipGetLastError());
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, h->stream, (const double *)h->partials, nblocks_of(h), h->sums_d);
+  hipLaunchKernelGGL(k_finalize, dim3(NSUM), dim3(1024), 0, h->stream, (const double *)h->partials, nblocks_of(h), h->sums_d);
   HIPCHK(h, hipGetLastError());
   double s[NSUM];
   HIPCHK(h, hipMemcpyAsync(s, h->sums_d, sizeof(s), hipMemcpyDeviceToHost, h->stream));
@@ -955,7 +1016,8 @@ int lbm_destroy(lbm_handle *h) {
   for (int b = 0; b < 2; ++b)
     if (h->buf[b]) hipFree(h->buf[b]);
   if (h->flags) hipFree(h->flags);
-  if (h->snap) hipFree(h->snap);
+  for (int b = 0; b < 2; ++b)
+    if (h->snap[b]) hipFree(h->snap[b]);
   if (h->partials) hipFree(h->partials);
   if (h->sums_d) hipFree(h->sums_d);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -974,13 +1036,13 @@ int lbm_copy_bench(int64_t nbytes, int32_t reps, double *ms) {
   HIPCHK(nullptr, hipMemset(a, 0, n * 16));
   hipStream_t s;
   HIPCHK(nullptr, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  const int grid = 256 * 32;
-  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a, b, n);
+  const long long grid = (n + 255) / 256;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, a, b, n);
   hipEvent_t e0, e1;
   HIPCHK(nullptr, hipEventCreate(&e0));
   HIPCHK(nullptr, hipEventCreate(&e1));
   HIPCHK(nullptr, hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, s, a, b, n);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, a, b, n);
   HIPCHK(nullptr, hipEventRecord(e1, s));
   HIPCHK(nullptr, hipEventSynchronize(e1));
   float t = 0;

@@ -1,0 +1,386 @@
+#include "solver.h"
+
+#include <sys/stat.h>
+
+#include <cstring>
+
+namespace lbmh {
+
+namespace {
+constexpr GDouble lbm_cs = 0.57735026918962576451;  // 1/sqrt(3), src/lbm/constants.h:26
+
+auto storageFromString(const std::string& s) -> int {
+  if (s == "f32" || s == "float32" || s == "fp32") return LBM_STORE_F32;
+  if (s == "f64" || s == "float64" || s == "fp64") return LBM_STORE_F64;
+  if (s == "f16" || s == "float16" || s == "fp16") return LBM_STORE_F16;
+  if (s == "bf16" || s == "bfloat16") return LBM_STORE_BF16;
+  TERMM(-1, "Invalid storage '" + s + "' (f32, f64, f16, bf16)");
+}
+
+void mkdirs(const std::string& path) {
+  std::string cur;
+  for (size_t i = 0; i < path.size(); ++i) {
+    cur += path[i];
+    if (path[i] == '/' || i + 1 == path.size()) mkdir(cur.c_str(), 0755);
+  }
+}
+}  // namespace
+
+// ------------------------------------------------------------- grid generator
+void GridGenerator::init(int, char**, std::string config_file) { m_config = Configuration::fromFile(config_file); }
+
+void GridGenerator::initBenchmark(int, char**) {
+  // the reference's generator benchmark has no configuration to run on
+  // (src/gridgenerator/gridGenerator.cpp:43-54,129); a fixed 3D box is used
+  m_config = {{"dim", 3},
+              {"maxNoCells", 1 << 24},
+              {"partitionLevel", 5},
+              {"uniformLevel", 5},
+              {"maxRfnmtLvl", 5},
+              {"geometry", {{"cube", {{"type", "cube"}, {"center", {0.0, 0.0, 0.0}}, {"length", 1}}}}}};
+}
+
+auto GridGenerator::run() -> GInt {
+  Configuration cfg(m_config);
+  m_grid = std::make_unique<CartesianGrid>();
+  m_grid->generate(cfg);
+  std::cout << "  * grid has " << m_grid->size() << " cells" << std::endl;
+  return 0;
+}
+
+// --------------------------------------------------------------- LBM solver
+LBMSolver::~LBMSolver() {
+  if (m_h) lbm_destroy(m_h);
+}
+
+void LBMSolver::check(int rc, const char* what) {
+  if (rc != LBM_OK) TERMM(-1, std::string(what) + ": " + lbm_last_error(m_h));
+}
+
+void LBMSolver::init(int, char**, std::string config_file) {
+  m_config = Configuration::fromFile(config_file);
+  if (!m_config.count("solver")) TERMM(-1, "No solver section in " + config_file);
+  m_solverCfg = std::make_unique<Configuration>(m_config["solver"], "solver.");
+}
+
+void LBMSolver::initBenchmark(int, char**) { m_benchmark = true; }
+
+void LBMSolver::transferGrid(const GridInterface& grid) {
+  const auto* cg = dynamic_cast<const CartesianGrid*>(&grid);
+  if (!cg) TERMM(-1, "Invalid grid type");
+  takeGrid(std::make_unique<CartesianGrid>(*cg));
+}
+
+void LBMSolver::takeGrid(std::unique_ptr<CartesianGrid> g) {
+  if (!m_solverCfg) TERMM(-1, "init() must be called before the grid is transferred");
+  m_grid = std::move(g);
+  std::cerr << "Transferring " << m_grid->dim() << "D Grid to LBM solver" << std::endl;
+  m_grid->loadGridInplace(*m_solverCfg);
+}
+
+// LBMSolver::loadConfiguration, src/lbm/solver.cpp:71-163
+void LBMSolver::loadConfiguration() {
+  Configuration& c = *m_solverCfg;
+  const std::string method = c.opt_config_value<std::string>("method", "bgk");
+  if (method != "bgk") TERMM(-1, "Invalid equation configuration!");
+  c.opt_config_value<std::string>("type", "lbm");
+  m_dim = m_grid->dim();
+  const std::string model = c.opt_config_value<std::string>("model", m_dim == 2 ? "D2Q9" : "D3Q19");
+  if (model == "D2Q9") m_lattice = 9;
+  else if (model == "D3Q19") m_lattice = 19;
+  else TERMM(-1, "Unsupported model " + model + " (D2Q9 and D3Q19 are built)");
+  if ((m_lattice == 9 && m_dim != 2) || (m_lattice == 19 && m_dim != 3))
+    TERMM(-1, "Invalid configuration the grid dimensionality is not matching!");
+  const std::string equation = c.opt_config_value<std::string>("equation", "navierstokes");
+  if (equation != "navierstokes") TERMM(-1, "Only the navierstokes equation is part of this build");
+  m_infoInterval = c.opt_config_value<GInt>("info_interval", 10);
+  m_convInterval = c.opt_config_value<GInt>("conv_interval", m_infoInterval);
+  if (m_convInterval < 1) TERMM(-1, "conv_interval must be >= 1");
+  m_keepAlive = c.opt_config_value<GInt>("keep_alive_time", 60);
+  m_maxSteps = c.required_config_value<GInt>("maxSteps");
+  m_outputDir = c.opt_config_value<std::string>("output_dir", "out/");
+  if (m_outputDir.empty() || m_outputDir.back() != '/') m_outputDir += '/';
+  m_solutionInterval = c.opt_config_value<GInt>("solution_interval", 100);
+  m_solutionFile = c.opt_config_value<std::string>("solution_filename", "solution");
+  c.getObject("cellFilter");
+  c.getObject("postprocessing");
+  c.opt_config_value<bool>("assumeAxisAligned", true);
+  m_refLength = c.opt_config_value<GDouble>("refLength", 1.0);
+  if (c.has_config_value("forcing"))
+    TERMM(-1, "The 'forcing' pressure-gradient hack is not part of this build; use the 'bodyForce' key (Guo forcing)");
+  const GDouble finestSpacing = 1.0 / (std::pow((GDouble)m_grid->size(), 1.0 / m_dim) - 1);
+  if (c.has_config_value("reynoldsnumber") && c.has_config_value("relaxation"))
+    TERMM(-1, "Only set either reynoldsnumber or relaxation");
+  m_ma = c.required_config_value<GDouble>("ma");
+  if (c.has_config_value("relaxation")) {
+    m_relaxTime = c.required_config_value<GDouble>("relaxation");
+    m_omega = 1.0 / m_relaxTime;
+    m_nu = (2 * m_relaxTime - 1) / 6.0;
+    m_re = m_ma * m_refLength / m_nu;
+  } else {
+    // src/lbm/solver.cpp:114-122 including its non-standard omega
+    m_re = c.required_config_value<GDouble>("reynoldsnumber");
+    m_nu = m_ma / m_re * m_refLength;
+    m_omega = 2.0 / (1.0 + 2.0 * m_nu * std::pow(2.0, (GDouble)m_grid->maxLvl()));
+    m_relaxTime = 1.0 / m_omega;
+  }
+  m_dt = finestSpacing * m_ma * lbm_cs / m_refLength;
+  m_convergence = c.opt_config_value<GDouble>("convergence", 1e-12);
+  // new keys (no reference counterpart)
+  m_storage = storageFromString(m_opt.storage.empty() ? c.opt_config_value<std::string>("storage", "f32") : m_opt.storage);
+  if (m_opt.storage.empty() == false) c.opt_config_value<std::string>("storage", "f32");
+  auto bf = c.opt_config_value<std::vector<GDouble>>("bodyForce", {0.0, 0.0, 0.0});
+  for (size_t d = 0; d < 3 && d < bf.size(); ++d) m_force[d] = bf[d];
+  // analytic-comparison keys are read in compareToAnalyticalResult
+  m_bnd.setupBndryCnds(c.getObject("boundary"), *m_grid);
+
+  std::cerr << "<<<<<<<<<<<<>>>>>>>>>>>>>\n"
+            << "LBM Type BGK\nLBM Model " << model << "\nEquation Navier-Stokes\n"
+            << "No. of variables " << m_dim + 1 << "\nNo. Leaf cells: " << m_grid->size()
+            << "\nMax Mesh Level: " << m_grid->maxLvl() << "\nNo. Bnd cells: " << m_grid->noBndCells()
+            << "\nRelaxation Time: " << m_relaxTime << "\nTimestep: " << m_dt << "\nOmega: " << m_omega
+            << "\nReynolds Number: " << m_re << "\nViscosity: " << m_nu << "\nReference V: " << m_ma * lbm_cs
+            << "\nStorage: " << (m_storage == LBM_STORE_F32 ? "f32" : m_storage == LBM_STORE_F64 ? "f64" : m_storage == LBM_STORE_F16 ? "f16" : "bf16")
+            << "\n+++++++++++++++++++++++++" << std::endl;
+}
+
+// initialCondition, src/lbm/solver.cpp:267-295: u from the boundary initCnd,
+// rho = 1 everywhere, f = feq(rho, u).
+void LBMSolver::initialCondition() {
+  const GInt N = m_grid->size();
+  const GInt nv = m_dim + 1;
+  m_vars.assign(N * nv, 0.0);
+  m_bnd.initCndBnd([&](GInt c, GInt v) -> GDouble& { return m_vars[c * nv + v]; });
+  for (GInt q = 0; q < N; ++q) m_vars[q * nv + m_dim] = 1.0;
+  const GInt S = m_grid->structSize();
+  std::vector<GDouble> rho(S, 1.0), u(S * m_dim, 0.0);
+  for (GInt q = 0; q < N; ++q) {
+    const GInt s = m_grid->structIndex(q);
+    rho[s] = m_vars[q * nv + m_dim];
+    for (GInt d = 0; d < m_dim; ++d) u[s * m_dim + d] = m_vars[q * nv + d];
+  }
+  check(lbm_init_fields(m_h, rho.data(), u.data()), "lbm_init_fields");
+}
+
+auto LBMSolver::convergenceCondition(const lbm_sums& s) -> bool {
+  static const char* names2[] = {"U", "V", "rho"};
+  static const char* names3[] = {"U", "V", "W", "rho"};
+  std::cerr << m_timeStep << ": ";
+  GDouble maxConv = 0;
+  bool bad = false;
+  for (GInt v = 0; v <= m_dim; ++v) {
+    const GDouble c = s.abs_diff[v];
+    std::cerr << "d" << (m_dim == 2 ? names2[v] : names3[v]) << "=" << c << " ";
+    if (std::isnan(c) || std::isinf(c)) bad = true;
+    maxConv = std::max(maxConv, c);
+  }
+  std::cerr << "res_rel=" << s.res_rel << std::endl;
+  if (m_timeStep > 1 && (bad || s.nonfinite > 0)) {
+    std::cerr << "Solution diverged!" << std::endl;
+    m_diverged = true;
+    return true;
+  }
+  if (m_timeStep > 1 && maxConv < m_convergence) {
+    std::cerr << "Reached convergence to: " << maxConv << std::endl;
+    return true;
+  }
+  return false;
+}
+
+void LBMSolver::fetchMacros() {
+  const GInt S = m_grid->structSize();
+  std::vector<GDouble> rho(S), u(S * m_dim);
+  check(lbm_get_macros(m_h, rho.data(), u.data()), "lbm_get_macros");
+  const GInt N = m_grid->size();
+  m_rho.assign(N, 0.0);
+  m_u.assign(N * m_dim, 0.0);
+  for (GInt q = 0; q < N; ++q) {
+    const GInt s = m_grid->structIndex(q);
+    m_rho[q] = rho[s];
+    for (GInt d = 0; d < m_dim; ++d) m_u[q * m_dim + d] = u[s * m_dim + d];
+  }
+}
+
+auto LBMSolver::run() -> GInt {
+  if (m_benchmark) return runBenchmark();
+  if (!m_grid) TERMM(-1, "No grid transferred to the LBM solver");
+  std::cerr << m_dim << "D LBM Solver started ||>" << std::endl;
+  loadConfiguration();
+  const DevicePlan plan = m_bnd.lower(*m_grid);
+  lbm_grid_desc g;
+  g.dim = (int32_t)m_dim;
+  g.nx = (int32_t)plan.nx;
+  g.ny = (int32_t)plan.ny;
+  g.nz = (int32_t)plan.nz;
+  g.solid = plan.anySolid ? plan.solid.data() : nullptr;
+  lbm_params p;
+  p.lattice = (int32_t)m_lattice;
+  p.storage = m_storage;
+  p.omega = m_omega;
+  for (int d = 0; d < 3; ++d) p.force[d] = m_force[d];
+  check(lbm_create(&g, &p, &m_h), "lbm_create");
+  check(lbm_set_bc(m_h, plan.faces.data(), (int32_t)plan.faces.size()), "lbm_set_bc");
+  initialCondition();
+
+  const auto t0 = std::chrono::steady_clock::now();
+  auto lastInfo = t0;
+  GInt lastStep = 0;
+  bool converged = false;
+  lbm_sums last{};
+  bool haveSums = false;
+  const GInt I = m_convInterval;
+  for (m_timeStep = 0; m_timeStep < m_maxSteps && !converged; ++m_timeStep) {
+    // writeInfo (src/lbm/solver.cpp:217-230)
+    if (m_timeStep > 0 && m_timeStep % m_infoInterval == 0) {
+      const auto now = std::chrono::steady_clock::now();
+      const double dt = std::chrono::duration<double>(now - lastInfo).count();
+      if (!m_opt.quiet) std::cerr << m_timeStep << "/" << m_maxSteps << " " << (m_timeStep - lastStep) / std::max(dt, 1e-9) << "it/s \n";
+      lastInfo = now;
+      lastStep = m_timeStep;
+    }
+    if (m_timeStep > 0 && m_timeStep % I == 0 && haveSums) converged = convergenceCondition(last);
+    // timeStep: launch k = t+1 gathers the fields of step k; the check at
+    // iteration t compares steps t-1 and t-2
+    const GInt k = m_timeStep + 1;
+    const bool chk = (k + 1) % I == 0;
+    const bool snap = (k + 2) % I == 0;
+    const int mode = chk ? (snap ? LBM_STEP_CHECK_SNAPSHOT : LBM_STEP_CHECK) : (snap ? LBM_STEP_SNAPSHOT : LBM_STEP_PLAIN);
+    lbm_sums s{};
+    const int rc = lbm_step_ex(m_h, mode, chk ? &s : nullptr);
+    if (rc == LBM_DIVERGED) {
+      last = s;
+      haveSums = true;
+    } else {
+      check(rc, "lbm_step_ex");
+      if (chk) {
+        last = s;
+        haveSums = true;
+      }
+    }
+  }
+  check(lbm_synchronize(m_h), "lbm_synchronize");
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
+            << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
+  fetchMacros();
+  if (m_diverged) {
+    writeSolution("bdiv");
+    TERMM(-1, "Solution diverged");
+  }
+  writeSolution("");
+  if (m_solverCfg->has_config_value("analyticalSolution")) compareToAnalyticalResult();
+  for (const auto& k : m_solverCfg->unusedConfigValues()) std::cerr << "WARNING: unused configuration value " << k << std::endl;
+  std::cerr << "LBM Solver finished <||" << std::endl;
+  return 0;
+}
+
+// compareToAnalyticalResult, src/lbm/solver.cpp:388-482 (couette2D_1_5 and
+// poiseuille2D_1 of src/lbm/analytical_solutions.h:20-59)
+void LBMSolver::compareToAnalyticalResult() {
+  Configuration& c = *m_solverCfg;
+  const std::string name = c.required_config_value<std::string>("analyticalSolution");
+  if (m_dim != 2) TERMM(-1, "Invalid analyticalSolution :" + name + " selected!");
+  GDouble dp = 0;
+  if (name == "poiseuille2D_1") dp = c.required_config_value<GDouble>("poiseuillePressureGradient");
+  auto solution = [&](const GDouble* x, GDouble* out) {
+    if (name == "couette2D_1_5") {
+      const GDouble reynoldsNum = 0.75, relaxTime = 0.9, refL = 1.0;
+      const GDouble dynViscosity = (2.0 * relaxTime - 1.0) / 6.0;
+      const GDouble refV = reynoldsNum * dynViscosity / refL;
+      out[0] = refV / 5.0 * x[1];
+      out[1] = 0;
+    } else if (name == "poiseuille2D_1") {
+      out[0] = dp / (2.0 * m_nu) * x[1] * (1.0 - 0.0 - x[1]);
+      out[1] = 0;
+    } else {
+      TERMM(-1, "Invalid analyticalSolution :" + name + " selected!");
+    }
+  };
+  std::set<GInt> excluded;
+  if (c.has_config_value("analyticalSolutionExcludeSurface")) {
+    for (const auto& s : c.required_config_value<std::vector<std::string>>("analyticalSolutionExcludeSurface"))
+      for (GInt q : m_grid->bndrySurface(s).getCellList()) excluded.insert(q);
+  }
+  GDouble maxError = 0, sumError = 0, sumErrorSq = 0, sumSolution = 0, sumSolutionSq = 0;
+  const GInt N = m_grid->size();
+  for (GInt q = 0; q < N; ++q) {
+    if (excluded.count(q)) continue;
+    GDouble x[2] = {m_grid->center(q, 0), m_grid->center(q, 1)}, a[2];
+    solution(x, a);
+    const GDouble dx = m_u[q * 2] - a[0], dy = m_u[q * 2 + 1] - a[1];
+    const GDouble delta = std::sqrt(dx * dx + dy * dy);
+    sumError += delta;
+    sumErrorSq += delta * delta;
+    sumSolution += std::sqrt(a[0] * a[0] + a[1] * a[1]);
+    sumSolutionSq += a[0] * a[0] + a[1] * a[1];
+    maxError = std::max(delta, maxError);
+  }
+  const GDouble L2 = (std::sqrt(sumErrorSq) / std::sqrt(sumSolutionSq)) / std::pow((GDouble)N, 1.0 / m_dim);
+  const GDouble gre = sumError / sumSolution;
+  std::cerr << "Comparing to analytical result " << name << "\nmax. Error: " << maxError << "\navg. L2: " << L2
+            << "\nglobal relative error: " << gre << std::endl;
+  const GDouble eMax = c.opt_config_value<GDouble>("errorMax", 1.0);
+  const GDouble eL2 = c.opt_config_value<GDouble>("errorL2", 1.0);
+  const GDouble eGre = c.opt_config_value<GDouble>("errorGRE", 1.0);
+  bool failed = false;
+  if (eMax < maxError) { failed = true; std::cerr << "Error bounds for the maximum error have failed!!! ( < " << eMax << ")" << std::endl; }
+  if (eL2 < L2 || std::isnan(L2)) { failed = true; std::cerr << "Error bounds for the L2 error have failed!!! ( < " << eL2 << ")" << std::endl; }
+  if (eGre < gre || std::isnan(gre)) { failed = true; std::cerr << "Error bounds for the global relative error have failed!!! ( < " << eGre << ")" << std::endl; }
+  if (failed) TERMM(-1, "Analytical testcase failed");
+}
+
+// Final macroscopic fields in reference cell-id order as CSV (the reference's
+// VTK point writer is not part of this build).
+void LBMSolver::writeSolution(const std::string& postfix) {
+  mkdirs(m_outputDir);
+  const std::string file = m_outputDir + m_solutionFile + "_" + std::to_string(m_timeStep) + postfix + ".csv";
+  std::ofstream out(file);
+  if (!out) {
+    std::cerr << "WARNING: cannot write " << file << std::endl;
+    return;
+  }
+  out.precision(17);
+  out << (m_dim == 2 ? "cellId,x,y,u,v,rho\n" : "cellId,x,y,z,u,v,w,rho\n");
+  for (GInt q = 0; q < m_grid->size(); ++q) {
+    out << q;
+    for (GInt d = 0; d < m_dim; ++d) out << "," << m_grid->center(q, d);
+    for (GInt d = 0; d < m_dim; ++d) out << "," << m_u[q * m_dim + d];
+    out << "," << m_rho[q] << "\n";
+  }
+}
+
+// `lbm --bench`: the reference's LBMSolver::initBenchmark is TERMM("Not
+// implemented!") (src/lbm/solver.cpp:39-46); here it times the D3Q19 BGK
+// periodic box 256^3 (Taylor-Green vortex u0 = 0.05) in FP32.
+auto LBMSolver::runBenchmark() -> GInt {
+  const int n = 256;
+  lbm_grid_desc g{3, n, n, n, nullptr};
+  lbm_params p{LBM_D3Q19, m_opt.storage.empty() ? LBM_STORE_F32 : storageFromString(m_opt.storage), 1.0 / 0.6, {0, 0, 0}};
+  check(lbm_create(&g, &p, &m_h), "lbm_create");
+  const GInt N = (GInt)n * n * n;
+  std::vector<GDouble> rho(N, 1.0), u(N * 3, 0.0);
+  const GDouble k = 2 * M_PI / n;
+  for (GInt z = 0; z < n; ++z)
+    for (GInt y = 0; y < n; ++y)
+      for (GInt x = 0; x < n; ++x) {
+        const GInt c = x + n * (y + n * z);
+        const GDouble X = (x + 0.5) * k, Y = (y + 0.5) * k, Z = (z + 0.5) * k;
+        u[c * 3 + 0] = 0.05 * std::sin(X) * std::cos(Y) * std::cos(Z);
+        u[c * 3 + 1] = -0.05 * std::cos(X) * std::sin(Y) * std::cos(Z);
+      }
+  check(lbm_init_fields(m_h, rho.data(), u.data()), "lbm_init_fields");
+  check(lbm_step(m_h, 20), "lbm_step");
+  double ms = 0;
+  const int steps = 200;
+  check(lbm_time_steps(m_h, steps, &ms), "lbm_time_steps");
+  int64_t ncell;
+  int32_t q, bpp;
+  lbm_info(m_h, &ncell, &q, &bpp);
+  const double s = ms * 1e-3 / steps;
+  std::cout << "D3Q19 BGK 256^3 periodic: " << ms / steps << " ms/step, " << N / s / 1e6 << " MLUPS, "
+            << 2.0 * q * bpp * N / s / 1e9 << " GB/s algorithmic (" << 2.0 * q * bpp * N / s / 8e12 * 100 << " % of 8 TB/s)"
+            << std::endl;
+  return 0;
+}
+
+}  // namespace lbmh

@@ -76,9 +76,10 @@ def test_d2q9_couette_moving_wall(storage, tol):
 @pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
 def test_d2q9_pressure_abb_channel(storage, tol):
     faces = dict(WALLS2)
-    faces["-x"] = ("pressure_abb", None, 1.001)
+    faces["-x"] = ("pressure_abb", None, 1.03)
     faces["+x"] = ("pressure_abb", None, 1.0)
-    eu, er, _ = run_pair((40, 16), 200, storage, 1 / 0.7, faces=faces)
+    N = 40 * 16
+    eu, er, _ = run_pair((40, 16), 300, storage, 1 / 0.7, faces=faces, init=(np.ones(N), np.zeros((N, 2))))
     assert eu < tol and er < tol
 
 
