@@ -74,12 +74,18 @@ typedef struct {
 
 /* One boundary face.  Replaces the per-surface LBMBndInterface objects that
  * LBMBndManager::setupBndryCnds builds (src/lbm/bnd/bnd.h:71-142).
- * value: wall/inlet velocity; rho: pressure (density) for pressure faces. */
+ * value: wall/inlet velocity; rho: pressure (density) for pressure faces.
+ * priority: a population that leaves the box across two faces (an edge or
+ * corner cell's diagonal) takes the rule of the face with the higher
+ * priority, ties going to the lower face index.  The reference applies its
+ * boundary objects in order and the last write wins (src/lbm/bnd/bnd.h:60-65),
+ * so a host passes the application order here. */
 typedef struct {
   int32_t face; /* 0..5 = -x,+x,-y,+y,-z,+z */
   int32_t kind; /* LBM_FACE_* */
   double value[3];
   double rho;
+  int32_t priority;
 } lbm_bc_desc;
 
 /* Reductions in FP64, summed in a fixed order (deterministic).

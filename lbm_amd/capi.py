@@ -31,7 +31,8 @@ class Params(ct.Structure):
 
 
 class BcDesc(ct.Structure):
-    _fields_ = [("face", ct.c_int32), ("kind", ct.c_int32), ("value", ct.c_double * 3), ("rho", ct.c_double)]
+    _fields_ = [("face", ct.c_int32), ("kind", ct.c_int32), ("value", ct.c_double * 3), ("rho", ct.c_double),
+                ("priority", ct.c_int32)]
 
 
 class Sums(ct.Structure):
@@ -95,6 +96,11 @@ class LBMError(RuntimeError):
         self.code = code
 
 
+def face_priority(kind):
+    """Default corner rule of the structured API: walls win over open faces."""
+    return 2 if kind == "wall" else 1
+
+
 def _dptr(a):
     return a.ctypes.data_as(ct.POINTER(ct.c_double))
 
@@ -102,8 +108,8 @@ def _dptr(a):
 class Lattice:
     """One structured lattice on the GPU (cell index x + nx*(y + ny*z)).
 
-    faces: dict like {"-y": ("wall", (0,0,0), 1.0), "+x": ("pressure_zh", None, 1.0)}
-    Unlisted faces are periodic.
+    faces: dict like {"-y": ("wall", (0,0,0), 1.0), "+x": ("pressure_zh", None, 1.0)}; an optional
+    fourth entry is the face priority (default: 2 for walls, 1 otherwise).  Unlisted faces are periodic.
     """
 
     def __init__(self, shape, lattice=None, storage="f32", omega=1.0, force=(0.0, 0.0, 0.0), faces=None,
@@ -139,10 +145,11 @@ class Lattice:
     def set_faces(self, faces):
         arr = (BcDesc * len(faces))()
         for k, (name, spec) in enumerate(faces.items()):
-            kind, val, rho = spec
+            kind, val, rho = spec[:3]
+            prio = spec[3] if len(spec) > 3 else face_priority(kind)
             val = tuple(val) if val is not None else (0.0, 0.0, 0.0)
             val = val + (0.0,) * (3 - len(val))
-            arr[k] = BcDesc(FACE_IDS[name], FACE[kind], (ct.c_double * 3)(*val), float(rho))
+            arr[k] = BcDesc(FACE_IDS[name], FACE[kind], (ct.c_double * 3)(*val), float(rho), int(prio))
         self._check(self.lib.lbm_set_bc(self.h, arr, len(faces)))
 
     def init(self, rho, u):

@@ -38,6 +38,7 @@ constexpr int NSUM = 12;  // mass, mom[3], absdiff[4], diff_sq, norm_sq, nonfini
 
 struct FaceDev {
   int kind;
+  int prio;
   double u[3];
   double rho;
 };
@@ -305,6 +306,11 @@ __global__ void __launch_bounds__(256) LBM_STEP_ATTR k_step(const KArgs a) {
       bool valid = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true)) &&
                    (cz > 0 ? vzm : (cz < 0 ? vzp : true));
       const long long s = (long long)sx + (long long)a.px * ((long long)sy + (long long)a.ny * sz);
+      // A population streams iff its upstream cell is fluid.  This is the
+      // symmetric rule; the reference composes diagonal neighbours from main
+      // ones (src/cartesiangrid.h:466-486), which next to staircase obstacles
+      // makes the neighbour relation one-sided and leaves some populations
+      // unwritten for a step.
       if (FLAGS && valid && j != Q - 1) valid = a.flags[s] == LBM_CELL_FLUID;
       const long long addr = valid ? (long long)j * plane + s : (long long)L::opp[j] * plane + cell;
       h[j] = NT ? Store<ST>::ld_nt(src + addr) : Store<ST>::ld(src + addr);
@@ -326,10 +332,10 @@ __global__ void __launch_bounds__(256) LBM_STEP_ATTR k_step(const KArgs a) {
         const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
         const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
         int f = -1;
-        // crossed faces: a wall wins, else the lowest axis
+        // crossed faces: the higher priority wins, ties to the lower axis
         if (ix) f = cx > 0 ? 0 : 1;
-        if (iy) { const int g = cy > 0 ? 2 : 3; if (f < 0 || (a.face[g].kind == LBM_FACE_WALL && a.face[f].kind != LBM_FACE_WALL)) f = g; }
-        if (iz) { const int g = cz > 0 ? 4 : 5; if (f < 0 || (a.face[g].kind == LBM_FACE_WALL && a.face[f].kind != LBM_FACE_WALL)) f = g; }
+        if (iy) { const int g = cy > 0 ? 2 : 3; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
+        if (iz) { const int g = cz > 0 ? 4 : 5; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
         if (f < 0) continue;  // solid obstacle: plain bounce-back, h[j] already holds S(x, opp j)
         const FaceDev &fd = a.face[f];
         const C wj = C(L::w[j]);
@@ -769,6 +775,7 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_PRESSURE_ZH) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
     seen[d.face] = true;
     face[d.face].kind = d.kind;
+    face[d.face].prio = d.priority;
     for (int k = 0; k < 3; ++k) face[d.face].u[k] = d.value[k];
     face[d.face].rho = d.rho;
   }

@@ -67,7 +67,7 @@ void LBMBndManager::setupBndryCnds(const json& bndConfig, CartesianGrid& grid) {
 
 namespace {
 auto sameRule(const BndRule& a, const BndRule& b) -> bool {
-  if (a.kind != b.kind) return false;
+  if (a.kind != b.kind || a.order != b.order) return false;
   for (int d = 0; d < 3; ++d)
     if (std::abs(a.value[d] - b.value[d]) > 1e-15) return false;
   return std::abs(a.rho - b.rho) <= 1e-15;
@@ -89,7 +89,9 @@ auto LBMBndManager::lower(const CartesianGrid& grid) const -> DevicePlan {
   }
   const std::array<GInt, 3> n{{plan.nx, plan.ny, plan.nz}};
   std::map<int, std::vector<BndRule>> rules;
+  int order = 0;
   for (const auto& b : m_bndrys) {
+    ++order;
     const Surface* s = b->surface();
     for (GInt c : s->getCellList()) {
       const GInt dir = s->dirOf(c);
@@ -97,7 +99,8 @@ auto LBMBndManager::lower(const CartesianGrid& grid) const -> DevicePlan {
       const GInt si = grid.structIndex(c);
       const GInt co[3] = {si % n[0], (si / n[0]) % n[1], si / (n[0] * n[1])};
       const bool onFace = (dir % 2 == 0) ? co[a] == 0 : co[a] == n[a] - 1;
-      const BndRule r = b->lower(c);
+      BndRule r = b->lower(c);
+      r.order = order;
       if (onFace) {
         rules[(int)dir].push_back(r);
       } else {
@@ -121,6 +124,8 @@ auto LBMBndManager::lower(const CartesianGrid& grid) const -> DevicePlan {
     d.kind = r0.kind;
     for (int k = 0; k < 3; ++k) d.value[k] = r0.value[k];
     d.rho = r0.rho;
+    // later boundary objects overwrite earlier ones in the reference
+    d.priority = r0.order;
     if (r0.kind != LBM_FACE_PERIODIC) plan.faces.push_back(d);
   }
   for (int a = 0; a < dim; ++a) {
@@ -146,6 +151,7 @@ auto LBMBndManager::lower(const CartesianGrid& grid) const -> DevicePlan {
       d.face = 2 * a + s;
       d.kind = LBM_FACE_WALL;
       d.rho = 1.0;
+      d.priority = 0;
       plan.faces.push_back(d);
     }
   }

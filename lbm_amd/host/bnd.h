@@ -22,6 +22,7 @@ using VarAccess = std::function<GDouble&(GInt cellId, GInt varId)>;
 
 struct BndRule {
   int kind = LBM_FACE_WALL;  // LBM_FACE_*
+  int order = 0;             // position of the boundary object in application order
   std::array<GDouble, 3> value{{0, 0, 0}};
   GDouble rho = 1.0;
   bool periodicViaBc = false;  // 'periodic' with generateBndry = true

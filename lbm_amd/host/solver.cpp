@@ -204,7 +204,7 @@ void LBMSolver::fetchMacros() {
 auto LBMSolver::run() -> GInt {
   if (m_benchmark) return runBenchmark();
   if (!m_grid) TERMM(-1, "No grid transferred to the LBM solver");
-  std::cerr << m_dim << "D LBM Solver started ||>" << std::endl;
+  std::cerr << m_grid->dim() << "D LBM Solver started ||>" << std::endl;
   loadConfiguration();
   const DevicePlan plan = m_bnd.lower(*m_grid);
   lbm_grid_desc g;

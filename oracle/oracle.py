@@ -9,8 +9,9 @@ boundary records applied after propagation in list order
 (src/lbm/bnd/bnd.h:60-65).
 
 Face rule for a direction whose downstream neighbour leaves the box: the
-crossed face decides the boundary; when two faces are crossed a wall wins,
-else the lower axis.  Missing neighbours inside the box (solid cells) are
+crossed face decides the boundary; when two faces are crossed the face with
+the higher priority wins (default: walls over open faces), ties to the lower
+axis.  Missing neighbours inside the box (solid cells) are
 plain no-slip bounce-back.  Zou-He records come after the per-direction ones.
 """
 import ctypes as ct
@@ -113,10 +114,12 @@ class Problem:
         self.Q = 9 if self.dim == 2 else 19
         self.omega = float(omega)
         self.force = np.array(list(force) + [0.0] * (3 - len(force)), dtype=np.float64)[:3]
-        self.faces = {k: (v[0], tuple(v[1]) if v[1] is not None else (0.0, 0.0, 0.0), float(v[2]))
+        self.faces = {k: (v[0], tuple(v[1]) if v[1] is not None else (0.0, 0.0, 0.0), float(v[2]),
+                          int(v[3]) if len(v) > 3 else (2 if v[0] == "wall" else 1))
                       for k, v in (faces or {}).items()}
         names = ["-x", "+x", "-y", "+y", "-z", "+z"]
         self.face_kind = [FACE[self.faces[n][0]] if n in self.faces else 0 for n in names]
+        self.face_prio = [self.faces[n][3] if n in self.faces else 0 for n in names]
         self.face_u = [self.faces[n][1] if n in self.faces else (0.0, 0.0, 0.0) for n in names]
         self.face_rho = [self.faces[n][2] if n in self.faces else 1.0 for n in names]
         self.periodic = [self.face_kind[2 * a] == 0 for a in range(3)]
@@ -148,9 +151,8 @@ class Problem:
                     t = t % ext[a]
                 else:
                     f = 2 * a + (1 if c[i, a] > 0 else 0)
-                    # a wall wins over other faces, else the lower axis
-                    take = out & ((face < 0) | ((self.face_kind[f] == FACE["wall"]) &
-                                                (np.array(self.face_kind)[np.maximum(face, 0)] != FACE["wall"])))
+                    # the higher face priority wins, ties to the lower axis
+                    take = out & ((face < 0) | (self.face_prio[f] > np.array(self.face_prio)[np.maximum(face, 0)]))
                     face = np.where(take, f, face)
                     ok &= ~out
                     t = np.clip(t, 0, ext[a] - 1)
@@ -320,3 +322,108 @@ def gaussian_splitmix(n, seed):
     out[0::2] = rad * np.cos(2 * np.pi * u2)
     out[1::2] = rad * np.sin(2 * np.pi * u2)
     return out[:n]
+
+
+class GridProblem(Problem):
+    """A reference-ordered grid (cell ids, neighbour table, surfaces) with the
+    reference's boundary objects applied in manager order
+    (LBMBndManager::apply, src/lbm/bnd/bnd.h:60-65).
+
+    nbr_table: [n][stride] grid neighbours (lattice direction i == grid
+    direction i for i < Q-1); surfaces: {name: (cells, dirs)}; bcs: list of
+    (surface name, spec dict) in application order, spec = the reference's
+    JSON boundary entry.  Gate of every boundary record: the cell's normal
+    n = dirVec(dir) and c_i . n >= eps (bnd_dirichlet.h:85-86).  With
+    textbook=True, missing directions no surface gate covers are bounced as
+    no-slip walls (interior obstacles); Zou-He records always come last.
+    """
+
+    def __init__(self, dim, nbr_table, surfaces, bcs, omega, force=(0.0, 0.0, 0.0), textbook=False):
+        self.dim = dim
+        self.Q = 9 if dim == 2 else 19
+        n = nbr_table.shape[0]
+        self.shape = (n, 1, 1)
+        self.omega = float(omega)
+        self.force = np.array(list(force) + [0.0] * (3 - len(force)), dtype=np.float64)[:3]
+        self.solid = np.zeros(n, dtype=bool)
+        c, w, opp = tables(self.Q)
+        self.c, self.w, self.opp = c, w, opp
+        self.nbr = np.ascontiguousarray(nbr_table[:, :self.Q - 1].astype(np.int64))
+        dv2 = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0)]
+        dv3 = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+        dv = dv2 if dim == 2 else dv3
+        recs, zh = [], []
+        gated = np.zeros((n, self.Q - 1), dtype=bool)
+        self.init_u = {}
+        for name, spec in bcs:
+            cells, dirs = surfaces[name]
+            t = spec["type"]
+            if t == "periodic" or not spec.get("generateBndry", True):
+                continue
+            for cidx, d in zip(cells, dirs):
+                nrm = np.array(dv[d], dtype=float)
+                mask = 0
+                for i in range(self.Q - 1):
+                    if c[i] @ nrm >= 2.220446049250313e-16:
+                        mask |= 1 << i
+                        gated[cidx, i] = True
+                r = OrcBC()
+                r.cell = int(cidx)
+                r.dirmask = mask
+                if t == "wall":
+                    r.kind = 1
+                    V = spec.get("tangentialVelocity", 0.0)
+                    r.u = (ct.c_double * 3)(V * nrm[1], V * nrm[0], 0.0)
+                    recs.append(r)
+                elif t == "dirichlet" and spec["model"] == "bounceback":
+                    r.kind = 1
+                    v = list(spec["value"])[:dim] + [0.0] * (3 - dim)
+                    r.u = (ct.c_double * 3)(*v)
+                    self.init_u[int(cidx)] = v[:dim]
+                    recs.append(r)
+                elif t == "pressure" and spec.get("model", "antibounceback") != "zouhe":
+                    r.kind = 2
+                    r.rho = spec["pressure"]
+                    inside = next(2 * a + (1 if nrm[a] < 0 else 0) for a in range(dim) if nrm[a] != 0)
+                    r.n1 = int(nbr_table[cidx, inside])
+                    r.n2 = int(nbr_table[r.n1, inside])
+                    recs.append(r)
+                elif (t == "dirichlet" and spec["model"] == "zouhe") or (t == "pressure" and spec.get("model") == "zouhe"):
+                    r.kind = 3 if t == "dirichlet" else 4
+                    r.axis = d // 2
+                    r.sign = 1 if d % 2 else -1
+                    if t == "dirichlet":
+                        v = list(spec["value"])[:dim] + [0.0] * (3 - dim)
+                        r.u = (ct.c_double * 3)(*v)
+                        self.init_u[int(cidx)] = v[:dim]
+                    else:
+                        r.rho = spec["pressure"]
+                    zh.append(r)
+                else:
+                    raise NotImplementedError(spec)
+        if textbook:
+            extra = []
+            for cidx in np.nonzero((self.nbr < 0).any(axis=1))[0]:
+                mask = 0
+                for i in range(self.Q - 1):
+                    if self.nbr[cidx, i] < 0 and not gated[cidx, i]:
+                        mask |= 1 << i
+                if mask:
+                    r = OrcBC()
+                    r.cell = int(cidx)
+                    r.kind = 1
+                    r.dirmask = mask
+                    extra.append(r)
+            recs = extra + recs
+        allr = recs + zh
+        self.nrec = len(allr)
+        self.recs = (OrcBC * max(1, len(allr)))(*allr)
+
+    def init_reference(self):
+        """initialCondition (src/lbm/solver.cpp:267-295): rho = 1, u from the
+        boundaries' initCnd, f = feq."""
+        n = self.shape[0]
+        u = np.zeros((n, self.dim))
+        for cidx, v in self.init_u.items():
+            u[cidx] = v
+        self.init(np.ones(n), u)

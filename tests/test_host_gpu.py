@@ -1,0 +1,152 @@
+"""The drop-in host on the GPU: the reference's own test configs through the
+`lbm` CLI, and the host's results against the FP64 oracle stepped on the
+same reference-ordered grid with the reference's boundary semantics."""
+import ctypes as ct
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.test_grid import host_grid
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LBM = os.path.join(ROOT, "lbm_amd", "bin", "lbm")
+REFCFG = os.path.join(ROOT, "tests", "golden", "reference_configs")
+OWNCFG = os.path.join(ROOT, "tests", "golden", "configs")
+
+
+def run_cli(cfg, tmp_path, *args, timeout=300):
+    p = tmp_path / "case.json"
+    cfg = json.loads(json.dumps(cfg))
+    cfg["solver"]["output_dir"] = str(tmp_path / "out")
+    p.write_text(json.dumps(cfg))
+    print("cli:", LBM, p, args, flush=True)
+    r = subprocess.run([LBM, str(p), *args], cwd=tmp_path, capture_output=True, text=True, timeout=timeout)
+    return r
+
+
+def test_couette_reference_thresholds_f64(tmp_path):
+    """test/couette/couette.json: periodic x, moving top wall (tangentialVelocity),
+    reynoldsnumber-based omega; must pass its own errorMax 4e-11 / errorL2 7e-10."""
+    cfg = json.load(open(os.path.join(REFCFG, "couette__couette.json")))
+    r = run_cli(cfg, tmp_path, "-p", "f64")
+    print(r.stderr[-3000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Reached convergence" in r.stderr
+    assert "max. Error" in r.stderr
+
+
+def test_couette_reference_f32_fails_loudly_only_on_thresholds(tmp_path):
+    """FP32 storage cannot reach the 4e-11 bound: the run must end through the
+    analytic check with status 255, like the reference's TERMM."""
+    cfg = json.load(open(os.path.join(REFCFG, "couette__couette.json")))
+    cfg["solver"]["maxSteps"] = 3000
+    r = run_cli(cfg, tmp_path, "-p", "f32")
+    assert r.returncode == 255
+    assert "Analytical testcase failed" in r.stderr
+
+
+def host_run(cfg, tmp_path, storage):
+    L = ct.CDLL(os.path.join(ROOT, "lbm_amd", "liblbm_host.so"))
+    L.lbm_host_run_file.argtypes = [ct.c_char_p, ct.c_char_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+    p = tmp_path / "case.json"
+    cfg["solver"]["output_dir"] = str(tmp_path / "out")
+    p.write_text(json.dumps(cfg))
+    h = host_grid(cfg)
+    n, dim = h["n"], h["dim"]
+    rho = np.zeros(n)
+    u = np.zeros(n * dim)
+    steps = ct.c_int64()
+    rc = L.lbm_host_run_file(str(p).encode(), storage.encode(), n, rho.ctypes.data, u.ctypes.data, ct.byref(steps))
+    assert rc == 0
+    return h, rho, u.reshape(n, dim), steps.value
+
+
+def oracle_run(cfg, h, steps, textbook=False):
+    bnd = cfg["solver"]["boundary"]
+    order = []
+    for g in sorted(bnd):
+        nb = len(bnd[g])
+        for f in sorted(bnd[g]):
+            order.append((f"{g}_{f}" if nb > 1 else g, bnd[g][f]))
+    s = cfg["solver"]
+    if "relaxation" in s:
+        omega = 1 / s["relaxation"]
+    else:
+        nu = s["ma"] / s["reynoldsnumber"] * s.get("refLength", 1.0)
+        omega = 2.0 / (1.0 + 2.0 * nu * 2.0 ** cfg["partitionLevel"])
+    nb = h["nb"]
+    if textbook or any(v.get("type") == "periodic" and v.get("generateBndry", True) for _, v in order):
+        # symmetric neighbours of the structured window (true diagonal cells)
+        nb = wrapped_neighbours(h, cfg)
+    p = O.GridProblem(h["dim"], nb, h["surf"], order, omega, force=s.get("bodyForce", (0, 0, 0)),
+                      textbook=textbook)
+    p.init_reference()
+    p.step(steps)
+    return p.macros()
+
+
+def wrapped_neighbours(h, cfg):
+    """'periodic' boundary objects (generateBndry = true) copy outgoing
+    populations to the linked cell of the other face (bnd_periodic.h:102-119),
+    i.e. the structured window wraps along that axis."""
+    nx, ny, nz = h["nxyz"]
+    dim = h["dim"]
+    ext = [nx, ny, nz]
+    sidx = h["sidx"]
+    cell_at = -np.ones(nx * ny * nz, dtype=np.int64)
+    cell_at[sidx] = np.arange(h["n"])
+    co = np.stack([sidx % nx, (sidx // nx) % ny, sidx // (nx * ny)], 1)
+    per = [False, False, False]
+    for g, faces in cfg["solver"]["boundary"].items():
+        for f, v in faces.items():
+            if v.get("type") == "periodic":
+                per["xyz".index(f[1])] = True
+    from oracle import grid_oracle as G
+    dv = G.DIR_VEC2 if dim == 2 else G.DIR_VEC3
+    nb = np.full_like(h["nb"], -1)
+    for d in range(h["nb"].shape[1]):
+        v = list(dv[d]) + [0] * (3 - len(dv[d]))
+        t = co + np.array(v)
+        ok = np.ones(h["n"], dtype=bool)
+        for a in range(3):
+            if per[a]:
+                t[:, a] %= ext[a]
+            else:
+                ok &= (t[:, a] >= 0) & (t[:, a] < ext[a])
+        s = np.where(ok, t[:, 0] + nx * (t[:, 1] + ny * t[:, 2]), 0)
+        nb[:, d] = np.where(ok, cell_at[s], -1)
+    return nb
+
+
+CASES = [
+    ("couette__couette.json", REFCFG, 400, False),
+    ("couette__couette_bnd_bbDirichlet.json", REFCFG, 400, False),
+    ("pressure_channel2d.json", OWNCFG, 400, False),
+    ("box3d_periodic_x.json", OWNCFG, 200, False),
+    ("sphere3d_channel.json", OWNCFG, 150, True),
+]
+
+
+@pytest.mark.parametrize("name,folder,steps,textbook", CASES)
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 1e-5)])
+def test_host_matches_oracle_on_reference_grid(name, folder, steps, textbook, storage, tol, tmp_path):
+    cfg = json.load(open(os.path.join(folder, name)))
+    s = cfg["solver"]
+    s["maxSteps"] = steps
+    s["convergence"] = 0.0
+    for k in ("analyticalSolution", "errorL2", "errorMax", "errorGRE", "analyticalSolutionExcludeSurface",
+              "postprocessing", "poiseuillePressureGradient"):
+        s.pop(k, None)
+    h, rho, u, done = host_run(cfg, tmp_path, storage)
+    assert done == steps
+    r_rho, r_u = oracle_run(cfg, h, steps, textbook)
+    eu = np.linalg.norm(u - r_u) / np.linalg.norm(r_u)
+    er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
+    print(f"{name} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
