@@ -149,8 +149,14 @@ int lbm_set_populations(lbm_handle *h, const double *f);
 int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms);
 
 /* Launch shape of the step kernel: block_x threads along x (multiple of 64),
- * block_y rows; nontemporal stores on/off.  0 keeps the current value. */
+ * block_y rows (0 keeps the current value); nontemporal loads/stores on (1)
+ * or off (0), -1 keeps the current setting. */
 int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nontemporal);
+
+/* Width of the population offsets of the step kernel: 32 (default whenever a
+ * population plane fits in 4 GiB; one 32-bit offset per access) or 64.
+ * Both give identical results; 32 is rejected for planes above 4 GiB. */
+int lbm_set_index_width(lbm_handle *h, int32_t bits);
 
 /* Size facts: number of cells, populations per cell, bytes per stored
  * population, and the name of the step kernel symbol family. */

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # packed FP32 instructions are disabled for every HIP translation unit
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-            "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
+            "-packed-fp32-ops", "-ffp-contract=off", "-Wno-unused-result", "-Wno-unused-value"]
 
 
 def run(cmd, cwd=None):
@@ -25,10 +25,29 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-def build_hip():
-    src = os.path.join(HERE, "csrc", "lbm_hip.hip")
-    out = os.path.join(HERE, "liblbm_hip.so")
-    run([HIPCC, *HIPFLAGS, "-shared", "-o", out + ".tmp", src])
+# translation units of liblbm_hip.so: the API and set-up kernels, and the
+# step kernels of each storage type (compiled in parallel)
+HIP_SOURCES = ["lbm_hip.hip", "lbm_step_f32.hip", "lbm_step_f64.hip", "lbm_step_f16.hip", "lbm_step_bf16.hip"]
+
+
+def build_hip(extra_flags=(), out=None, jobs=None):
+    """Compile every HIP translation unit for gfx950 and link liblbm_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = out or os.path.join(HERE, "liblbm_hip.so")
+    objdir = os.path.join(os.path.dirname(out), "_build")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [*HIPFLAGS, *extra_flags]
+
+    def compile_one(name):
+        obj = os.path.join(objdir, name.replace(".hip", ".o"))
+        run([HIPCC, *flags, "-c", "-o", obj, os.path.join(HERE, "csrc", name)])
+        return obj
+
+    jobs = jobs or min(len(HIP_SOURCES), os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, HIP_SOURCES))
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
     return out
 

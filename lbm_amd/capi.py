@@ -49,7 +49,7 @@ class Sums(ct.Structure):
 # every symbol declared in include/lbm_hip.h
 EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_get_macros",
            "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_set_launch",
-           "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
+           "lbm_set_index_width", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
 _lib = None
 
@@ -76,6 +76,7 @@ def load():
     lib.lbm_set_populations.argtypes = [P, dp]
     lib.lbm_time_steps.argtypes = [P, ct.c_int32, dp]
     lib.lbm_set_launch.argtypes = [P, ct.c_int32, ct.c_int32, ct.c_int32]
+    lib.lbm_set_index_width.argtypes = [P, ct.c_int32]
     lib.lbm_info.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]
     lib.lbm_synchronize.argtypes = [P]
     lib.lbm_destroy.argtypes = [P]
@@ -204,6 +205,9 @@ class Lattice:
     def set_launch(self, bx=0, by=0, nt=-1):
         self._check(self.lib.lbm_set_launch(self.h, int(bx), int(by), int(nt)))
 
+    def set_index_width(self, bits):
+        self._check(self.lib.lbm_set_index_width(self.h, int(bits)))
+
     def sync(self):
         self._check(self.lib.lbm_synchronize(self.h))
 
@@ -226,3 +230,145 @@ def copy_bench(nbytes, reps=20):
     if rc != LBM_OK:
         raise LBMError(rc, lib.lbm_last_error(None).decode())
     return ms.value
+
+
+# ---------------------------------------------------------------- batches
+class DomainDesc(ct.Structure):
+    _fields_ = [("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32), ("omega", ct.c_double),
+                ("solid", ct.POINTER(ct.c_uint8)), ("faces", BcDesc * 6), ("nfaces", ct.c_int32)]
+
+
+class DomainResult(ct.Structure):
+    _fields_ = [("steps", ct.c_int64), ("res_rel", ct.c_double), ("mass", ct.c_double)]
+
+
+BATCH_EXPORTS = ["lbm_batch_create", "lbm_batch_tune", "lbm_batch_set_tiling", "lbm_batch_init_fields",
+                 "lbm_batch_run", "lbm_batch_get_macros", "lbm_batch_get_populations", "lbm_batch_info",
+                 "lbm_batch_last_error", "lbm_batch_destroy"]
+_batch_ready = False
+
+
+def _load_batch():
+    global _batch_ready
+    lib = load()
+    if _batch_ready:
+        return lib
+    P = ct.c_void_p
+    dp = ct.POINTER(ct.c_double)
+    lib.lbm_batch_create.argtypes = [ct.c_int32, ct.c_int32, ct.POINTER(DomainDesc), ct.c_int32, ct.POINTER(P)]
+    lib.lbm_batch_tune.argtypes = [P, ct.c_int32, ct.c_char_p, ct.c_int32]
+    lib.lbm_batch_set_tiling.argtypes = [P, ct.c_int32, ct.c_int32]
+    lib.lbm_batch_init_fields.argtypes = [P, dp, dp]
+    lib.lbm_batch_run.argtypes = [P, ct.c_int32, ct.c_double, ct.c_int64, ct.POINTER(DomainResult), dp]
+    lib.lbm_batch_get_macros.argtypes = [P, ct.c_int32, dp, dp]
+    lib.lbm_batch_get_populations.argtypes = [P, ct.c_int32, dp]
+    lib.lbm_batch_info.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int32),
+                                   ct.POINTER(ct.c_int32)]
+    lib.lbm_batch_last_error.argtypes = [P]
+    lib.lbm_batch_last_error.restype = ct.c_char_p
+    lib.lbm_batch_destroy.argtypes = [P]
+    for n in BATCH_EXPORTS:
+        if n != "lbm_batch_last_error":
+            getattr(lib, n).restype = ct.c_int
+    _batch_ready = True
+    return lib
+
+
+class Batch:
+    """Many uneven structured domains advanced together (include/lbm_batch.h).
+
+    domains: list of dicts {shape: (nx, ny[, nz]), omega, solid (optional flat bool array),
+    faces: {"-x": (kind, u, rho[, priority]), ...}}."""
+
+    def __init__(self, domains, storage="f32"):
+        lib = _load_batch()
+        self.lib = lib
+        self.storage = storage
+        shapes = [tuple(d["shape"]) + ((1,) if len(d["shape"]) == 2 else ()) for d in domains]
+        self.dim = 2 if shapes[0][2] == 1 else 3
+        self.Q = D2Q9 if self.dim == 2 else D3Q19
+        self.shapes = shapes
+        arr = (DomainDesc * len(domains))()
+        self._keep = []
+        for k, d in enumerate(domains):
+            nx, ny, nz = shapes[k]
+            arr[k].nx, arr[k].ny, arr[k].nz = nx, ny, nz
+            arr[k].omega = float(d["omega"])
+            if d.get("solid") is not None:
+                s = np.ascontiguousarray(np.asarray(d["solid"], dtype=np.uint8).reshape(-1))
+                assert s.size == nx * ny * nz
+                self._keep.append(s)
+                arr[k].solid = s.ctypes.data_as(ct.POINTER(ct.c_uint8))
+            faces = d.get("faces") or {}
+            for i, (name, spec) in enumerate(faces.items()):
+                kind, val, rho = spec[:3]
+                prio = spec[3] if len(spec) > 3 else face_priority(kind)
+                val = tuple(val) if val is not None else (0.0, 0.0, 0.0)
+                val = val + (0.0,) * (3 - len(val))
+                arr[k].faces[i] = BcDesc(FACE_IDS[name], FACE[kind], (ct.c_double * 3)(*val), float(rho), int(prio))
+            arr[k].nfaces = len(faces)
+        h = ct.c_void_p()
+        rc = lib.lbm_batch_create(self.Q, STORE[storage], arr, len(domains), ct.byref(h))
+        if rc != LBM_OK:
+            raise LBMError(rc, lib.lbm_batch_last_error(None).decode())
+        self.h = h
+        self.ndom = len(domains)
+        self.ncells = [s[0] * s[1] * s[2] for s in shapes]
+
+    def _check(self, rc):
+        if rc != LBM_OK:
+            raise LBMError(rc, self.lib.lbm_batch_last_error(self.h).decode())
+
+    def tune(self, steps=20):
+        buf = ct.create_string_buffer(64)
+        self._check(self.lib.lbm_batch_tune(self.h, int(steps), buf, 64))
+        tx, blocks, ms = buf.value.decode().split(",")
+        return int(tx), int(blocks), float(ms)
+
+    def set_tiling(self, tx, blocks):
+        self._check(self.lib.lbm_batch_set_tiling(self.h, int(tx), int(blocks)))
+
+    def init(self, rho, u):
+        rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1))
+        assert rho.size == sum(self.ncells) and u.size == rho.size * self.dim
+        self._check(self.lib.lbm_batch_init_fields(self.h, _dptr(rho), _dptr(u)))
+
+    def run(self, interval, tol, max_steps):
+        res = (DomainResult * self.ndom)()
+        ms = ct.c_double()
+        self._check(self.lib.lbm_batch_run(self.h, int(interval), float(tol), int(max_steps), res, ct.byref(ms)))
+        steps = np.array([r.steps for r in res], dtype=np.int64)
+        rr = np.array([r.res_rel for r in res])
+        mass = np.array([r.mass for r in res])
+        return steps, rr, mass, ms.value
+
+    def macros(self, dom):
+        n = self.ncells[dom]
+        rho = np.empty(n)
+        u = np.empty(n * self.dim)
+        self._check(self.lib.lbm_batch_get_macros(self.h, int(dom), _dptr(rho), _dptr(u)))
+        return rho, u.reshape(n, self.dim)
+
+    def populations(self, dom):
+        n = self.ncells[dom]
+        f = np.empty(self.Q * n)
+        self._check(self.lib.lbm_batch_get_populations(self.h, int(dom), _dptr(f)))
+        return f.reshape(self.Q, n)
+
+    def info(self):
+        nc, nt = ct.c_int64(), ct.c_int64()
+        tx, bl = ct.c_int32(), ct.c_int32()
+        self._check(self.lib.lbm_batch_info(self.h, ct.byref(nc), ct.byref(nt), ct.byref(tx), ct.byref(bl)))
+        return dict(cells=nc.value, tiles=nt.value, tile_x=tx.value, blocks=bl.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

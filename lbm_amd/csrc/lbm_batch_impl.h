@@ -1,0 +1,525 @@
+// Batched uneven domains (include/lbm_batch.h).  Included at the end of
+// lbm_hip.hip so that it shares the single-domain set-up kernels; the batch
+// step kernel lives with the other step kernels (lbm_step_kernels.h) and the
+// per-domain table (DomainDev, BatchArgs) in lbm_launch.h.
+//
+// Layout: every domain owns Q population planes of its own padded size in
+// two shared buffers (A-B), a flag block and a snapshot block; a table of
+// DomainDev in device memory holds offsets and parameters.  Work is a flat
+// list of tiles (domain, x0, y0, z), each tile = 256 cells; the step kernel
+// loops over tiles with a grid-stride loop, so one launch is either
+// persistent (a few blocks per CU) or one block per tile -- the autotuner
+// picks.  The check step reduces per tile into structure-of-arrays partials
+// and k_domain_finalize sums each domain's contiguous tile range in order
+// (deterministic), compares with the tolerance and retires the domain.
+
+namespace lbm {
+
+// One thread per domain: sum its tiles' partials in order, update the
+// residual, retire converged or diverged domains.
+__global__ void k_domain_finalize(const DomainDev *doms, int ndom, const double *partials, long long ntiles,
+                                  int *active, double *res, double *mass, long long *conv_cycle, const long long *cycle,
+                                  double tol) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndom || !active[d]) return;
+  double diff = 0, norm = 0, m = 0, bad = 0;
+  for (int t = doms[d].tile_begin; t < doms[d].tile_end; ++t) {
+    m += partials[0 * ntiles + t];
+    diff += partials[8 * ntiles + t];
+    norm += partials[9 * ntiles + t];
+    bad += partials[10 * ntiles + t];
+  }
+  const double r = norm > 0 ? sqrt(diff / norm) : (diff > 0 ? INFINITY : 0.0);
+  res[d] = r;
+  mass[d] = m;
+  if (bad > 0 || !isfinite(r)) {
+    active[d] = 0;
+    conv_cycle[d] = -2;
+  } else if (r < tol) {
+    active[d] = 0;
+    conv_cycle[d] = *cycle;
+  }
+}
+
+__global__ void k_tick(long long *cycle) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *cycle += 1;
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_ll(long long *p, int n, long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace lbm
+
+struct lbm_batch {
+  int Q = 0, dim = 0, storage = 0, esize = 0;
+  int ndom = 0;
+  std::vector<DomainDev> doms;
+  std::vector<int> ncells;  // compact cells per domain
+  std::vector<std::vector<uint8_t>> solid;  // host copy of each domain's solid mask (compact)
+  std::vector<int> active_h;                // host view of the active flags at the last tiling
+  long long total_pop = 0, total_cells = 0, compact_cells = 0;
+  void *buf[2] = {nullptr, nullptr};
+  void *snap[2] = {nullptr, nullptr};
+  uint8_t *flags = nullptr;
+  DomainDev *doms_d = nullptr;
+  int4 *tiles_d = nullptr;
+  long long ntiles = 0;
+  int tx = 64, blocks = 0;  // blocks == 0: one block per tile
+  int *active_d = nullptr;
+  double *res_d = nullptr, *mass_d = nullptr, *partials = nullptr;
+  long long *conv_d = nullptr, *cycle_d = nullptr;
+  hipStream_t stream = nullptr;
+  int cur = 0;
+  hipGraphExec_t graph = nullptr;
+  int graph_interval = 0;
+  double graph_tol = -1;
+  std::string err;
+};
+
+static int bfail(lbm_batch *b, int code, const std::string &msg) {
+  if (b) b->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, call)                                                                     \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return bfail((b), LBM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// Tiles of every still-active domain.  A tile needs the flag field when a
+// solid cell lies within one cell of it (its pull sources or itself).
+static int batch_build_tiles(lbm_batch *b, int tx) {
+  const int ty = 256 / tx;
+  std::vector<int4> tiles;
+  for (int d = 0; d < b->ndom; ++d) {
+    DomainDev &D = b->doms[d];
+    D.tile_begin = (int)tiles.size();
+    if (!b->active_h.empty() && !b->active_h[d]) {
+      D.tile_end = D.tile_begin;
+      continue;
+    }
+    const std::vector<uint8_t> &sol = b->solid[d];
+    for (int z = 0; z < D.nz; ++z)
+      for (int y0 = 0; y0 < D.ny; y0 += ty)
+        for (int x0 = 0; x0 < D.nx; x0 += tx) {
+          int need = 0;
+          if (!sol.empty()) {
+            for (int zz = std::max(0, z - 1); zz <= std::min(D.nz - 1, z + 1) && !need; ++zz)
+              for (int yy = std::max(0, y0 - 1); yy <= std::min(D.ny - 1, y0 + ty) && !need; ++yy)
+                for (int xx = std::max(0, x0 - 1); xx <= std::min(D.nx - 1, x0 + tx); ++xx)
+                  if (sol[xx + (size_t)D.nx * (yy + (size_t)D.ny * zz)]) {
+                    need = 1;
+                    break;
+                  }
+            // periodic wrap reaches the opposite edge
+            if (!need && (D.per[0] || D.per[1] || D.per[2])) need = 1;
+          }
+          tiles.push_back(make_int4(d, x0, y0, z | (need << 30)));
+        }
+    D.tile_end = (int)tiles.size();
+  }
+  if (tiles.empty()) tiles.push_back(make_int4(0, 0, 0, 0));
+  if (b->tiles_d) BHIPCHK(b, hipFree(b->tiles_d));
+  b->tiles_d = nullptr;
+  b->ntiles = (long long)tiles.size();
+  BHIPCHK(b, hipMalloc(&b->tiles_d, tiles.size() * sizeof(int4)));
+  BHIPCHK(b, hipMemcpy(b->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  BHIPCHK(b, hipMemcpy(b->doms_d, b->doms.data(), b->doms.size() * sizeof(DomainDev), hipMemcpyHostToDevice));
+  if (b->partials) BHIPCHK(b, hipFree(b->partials));
+  b->partials = nullptr;
+  BHIPCHK(b, hipMalloc(&b->partials, (size_t)b->ntiles * NSUM * sizeof(double)));
+  b->tx = tx;
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return LBM_OK;
+}
+
+static BatchArgs batch_args(lbm_batch *b, int parity, int mode) {
+  BatchArgs a;
+  memset(&a, 0, sizeof(a));
+  a.src = b->buf[parity];
+  a.dst = b->buf[parity ^ 1];
+  a.flags = b->flags;
+  a.doms = b->doms_d;
+  a.tiles = b->tiles_d;
+  a.ntiles = (int)b->ntiles;
+  a.tx = b->tx;
+  a.active = b->active_d;
+  if (mode >= 1) {
+    // snapshot written by the step before the check, read by the check
+    a.snap_in = b->snap[0];
+    a.snap_out = b->snap[0];
+    a.partials = b->partials;
+  }
+  return a;
+}
+
+static int batch_grid(const lbm_batch *b) { return b->blocks > 0 ? b->blocks : (int)b->ntiles; }
+
+static void batch_launch(lbm_batch *b, int parity, int mode) {
+  const BatchArgs a = batch_args(b, parity, mode);
+  dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b), b->stream, mode);
+}
+
+extern "C" {
+
+int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *doms, int32_t ndom, lbm_batch **out) {
+  if (!doms || !out || ndom <= 0) return bfail(nullptr, LBM_EINVAL, "lbm_batch_create: bad argument");
+  *out = nullptr;
+  if (lattice != LBM_D2Q9 && lattice != LBM_D3Q19) return bfail(nullptr, LBM_EINVAL, "lbm_batch_create: lattice 9 or 19");
+  const int es = esize_of(storage);
+  if (!es) return bfail(nullptr, LBM_EINVAL, "lbm_batch_create: unknown storage");
+  lbm_batch *b = new lbm_batch();
+  b->Q = lattice;
+  b->dim = lattice == LBM_D2Q9 ? 2 : 3;
+  b->storage = storage;
+  b->esize = es;
+  b->ndom = ndom;
+  auto bail = [&](int code) {
+    lbm_batch_destroy(b);
+    return code;
+  };
+  std::vector<uint8_t> flags;
+  long long pop = 0, cells = 0, compact = 0;
+  for (int d = 0; d < ndom; ++d) {
+    const lbm_domain_desc &dd = doms[d];
+    if (dd.nx < 3 || dd.ny < 3 || dd.nz < 1 || (b->dim == 2 && dd.nz != 1) || (b->dim == 3 && dd.nz < 3)) {
+      bfail(nullptr, LBM_EINVAL, "lbm_batch_create: domain " + std::to_string(d) + " has an invalid size");
+      return bail(LBM_EINVAL);
+    }
+    if (!(dd.omega > 0 && dd.omega < 2)) {
+      bfail(nullptr, LBM_EINVAL, "lbm_batch_create: omega must be in (0,2)");
+      return bail(LBM_EINVAL);
+    }
+    DomainDev D;
+    memset(&D, 0, sizeof(D));
+    D.nx = dd.nx;
+    D.ny = dd.ny;
+    D.nz = dd.nz;
+    D.px = (dd.nx + 15) / 16 * 16;
+    D.plane = (long long)D.px * D.ny * D.nz;
+    if ((unsigned long long)D.plane * (unsigned long long)es > 0xffffffffull) {
+      bfail(nullptr, LBM_EINVAL, "lbm_batch_create: domain " + std::to_string(d) +
+                                     " is too large for a batch (a population plane above 4 GiB); run it alone");
+      return bail(LBM_EINVAL);
+    }
+    D.pop_off = pop;
+    D.cell_off = cells;
+    D.omega = dd.omega;
+    for (int f = 0; f < 6; ++f) {
+      D.face[f].kind = LBM_FACE_PERIODIC;
+      D.face[f].rho = 1.0;
+    }
+    for (int i = 0; i < dd.nfaces; ++i) {
+      const lbm_bc_desc &f = dd.faces[i];
+      if (f.face < 0 || f.face >= 2 * b->dim || f.kind < 0 || f.kind > LBM_FACE_PRESSURE_ZH) {
+        bfail(nullptr, LBM_EINVAL, "lbm_batch_create: bad face in domain " + std::to_string(d));
+        return bail(LBM_EINVAL);
+      }
+      D.face[f.face].kind = f.kind;
+      D.face[f.face].prio = f.priority;
+      for (int k = 0; k < 3; ++k) D.face[f.face].u[k] = f.value[k];
+      D.face[f.face].rho = f.rho;
+    }
+    for (int ax = 0; ax < 3; ++ax) {
+      const bool p0 = D.face[2 * ax].kind == LBM_FACE_PERIODIC, p1 = D.face[2 * ax + 1].kind == LBM_FACE_PERIODIC;
+      if (ax < b->dim && p0 != p1) {
+        bfail(nullptr, LBM_EINVAL, "lbm_batch_create: periodic face without partner");
+        return bail(LBM_EINVAL);
+      }
+      D.per[ax] = p0 ? 1 : 0;
+    }
+    const long long nc = D.plane;
+    flags.resize(cells + nc, LBM_CELL_SOLID);
+    for (int z = 0; z < D.nz; ++z)
+      for (int y = 0; y < D.ny; ++y)
+        for (int x = 0; x < D.nx; ++x) {
+          const uint8_t v = dd.solid ? dd.solid[x + (long long)D.nx * (y + (long long)D.ny * z)] : LBM_CELL_FLUID;
+          flags[cells + x + (long long)D.px * (y + (long long)D.ny * z)] = v ? LBM_CELL_SOLID : LBM_CELL_FLUID;
+        }
+    b->doms.push_back(D);
+    b->ncells.push_back(D.nx * D.ny * D.nz);
+    std::vector<uint8_t> sol;
+    if (dd.solid) sol.assign(dd.solid, dd.solid + (size_t)D.nx * D.ny * D.nz);
+    b->solid.push_back(std::move(sol));
+    pop += (long long)b->Q * D.plane;
+    cells += nc;
+    compact += (long long)D.nx * D.ny * D.nz;
+  }
+  b->total_pop = pop;
+  b->total_cells = cells;
+  b->compact_cells = compact;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "stream"));
+  for (int k = 0; k < 2; ++k) {
+    if (hipMalloc(&b->buf[k], (size_t)pop * es) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "population buffers"));
+    if (hipMemset(b->buf[k], 0, (size_t)pop * es) != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "memset"));
+  }
+  const size_t cs = snap_bytes(storage);
+  if (hipMalloc(&b->snap[0], (size_t)cells * cs) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "snapshot"));
+  if (hipMalloc(&b->flags, flags.size()) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "flags"));
+  if (hipMemcpy(b->flags, flags.data(), flags.size(), hipMemcpyHostToDevice) != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "flags"));
+  if (hipMalloc(&b->doms_d, ndom * sizeof(DomainDev)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "domain table"));
+  if (hipMalloc(&b->active_d, ndom * sizeof(int)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "active"));
+  if (hipMalloc(&b->res_d, ndom * sizeof(double)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "res"));
+  if (hipMalloc(&b->mass_d, ndom * sizeof(double)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "mass"));
+  if (hipMalloc(&b->conv_d, ndom * sizeof(long long)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "conv"));
+  if (hipMalloc(&b->cycle_d, sizeof(long long)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "cycle"));
+  hipLaunchKernelGGL(k_fill_int, dim3((ndom + 255) / 256), dim3(256), 0, b->stream, b->active_d, ndom, 1);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "init"));
+  int rc = batch_build_tiles(b, 64);
+  if (rc) return bail(rc);
+  *out = b;
+  return LBM_OK;
+}
+
+int lbm_batch_set_tiling(lbm_batch *b, int32_t tile_x, int32_t blocks) {
+  if (!b || !(tile_x == 32 || tile_x == 64 || tile_x == 128 || tile_x == 256) || blocks < 0)
+    return bfail(b, LBM_EINVAL, "lbm_batch_set_tiling: tile_x in {32,64,128,256}, blocks >= 0");
+  b->blocks = blocks;
+  return batch_build_tiles(b, tile_x);
+}
+
+int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t choice_len) {
+  if (!b || steps_per_trial < 1) return bfail(b, LBM_EINVAL, "lbm_batch_tune: bad argument");
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
+  }
+  hipEvent_t e0, e1;
+  BHIPCHK(b, hipEventCreate(&e0));
+  BHIPCHK(b, hipEventCreate(&e1));
+  double best = 1e30;
+  int best_tx = 64, best_blocks = 0;
+  const int txs[4] = {32, 64, 128, 256};
+  const int per_cu[4] = {0, 2, 4, 16};
+  for (int ti = 0; ti < 4; ++ti) {
+    int rc = batch_build_tiles(b, txs[ti]);
+    if (rc) return rc;
+    for (int gi = 0; gi < 4; ++gi) {
+      b->blocks = per_cu[gi] * ncu;
+      for (int w = 0; w < 2; ++w) batch_launch(b, (b->cur + w) & 1, 0);
+      BHIPCHK(b, hipEventRecord(e0, b->stream));
+      for (int s = 0; s < steps_per_trial; ++s) batch_launch(b, (b->cur + s) & 1, 0);
+      BHIPCHK(b, hipEventRecord(e1, b->stream));
+      BHIPCHK(b, hipEventSynchronize(e1));
+      float t = 0;
+      BHIPCHK(b, hipEventElapsedTime(&t, e0, e1));
+      if (t < best) {
+        best = t;
+        best_tx = txs[ti];
+        best_blocks = b->blocks;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->blocks = best_blocks;
+  int rc = batch_build_tiles(b, best_tx);
+  if (rc) return rc;
+  if (choice && choice_len > 0) snprintf(choice, choice_len, "%d,%d,%.4f", best_tx, best_blocks, best / steps_per_trial);
+  return LBM_OK;
+}
+
+int lbm_batch_init_fields(lbm_batch *b, const double *rho, const double *u) {
+  if (!b || !rho || !u) return bfail(b, LBM_EINVAL, "lbm_batch_init_fields: null argument");
+  double *rd = nullptr, *ud = nullptr;
+  BHIPCHK(b, hipMalloc(&rd, b->compact_cells * sizeof(double)));
+  BHIPCHK(b, hipMalloc(&ud, b->compact_cells * b->dim * sizeof(double)));
+  BHIPCHK(b, hipMemcpy(rd, rho, b->compact_cells * sizeof(double), hipMemcpyHostToDevice));
+  BHIPCHK(b, hipMemcpy(ud, u, b->compact_cells * b->dim * sizeof(double), hipMemcpyHostToDevice));
+  long long coff = 0;
+  b->cur = 0;
+  if (!b->active_h.empty()) {
+    b->active_h.clear();
+    const int rc = batch_build_tiles(b, b->tx);
+    if (rc) return rc;
+  }
+  for (int d = 0; d < b->ndom; ++d) {
+    const DomainDev &D = b->doms[d];
+    KArgs a;
+    memset(&a, 0, sizeof(a));
+    a.dst = static_cast<char *>(b->buf[0]) + D.pop_off * b->esize;
+    a.flags = b->flags + D.cell_off;
+    a.plane = D.plane;
+    a.nx = D.nx; a.ny = D.ny; a.nz = D.nz; a.px = D.px;
+    a.omega = D.omega;
+    a.snap_out = nullptr;
+    dim3 g((D.nx + 63) / 64, (D.ny + 3) / 4, D.nz), bl(64, 4, 1);
+    dispatch_qs<InitL>(b->Q, b->storage, a, g, bl, b->stream, (const double *)(rd + coff),
+                       (const double *)(ud + coff * b->dim));
+    coff += b->ncells[d];
+  }
+  BHIPCHK(b, hipGetLastError());
+  hipLaunchKernelGGL(k_fill_int, dim3((b->ndom + 255) / 256), dim3(256), 0, b->stream, b->active_d, b->ndom, 1);
+  hipLaunchKernelGGL(k_fill_ll, dim3((b->ndom + 255) / 256), dim3(256), 0, b->stream, b->conv_d, b->ndom, -1LL);
+  hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, b->stream, b->cycle_d, 1, 0LL);
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  BHIPCHK(b, hipFree(rd));
+  BHIPCHK(b, hipFree(ud));
+  return LBM_OK;
+}
+
+static int batch_capture(lbm_batch *b, int interval, double tol) {
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  hipGraph_t g;
+  BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
+  for (int s = 0; s < interval; ++s) {
+    const int mode = s == interval - 2 ? 1 : (s == interval - 1 ? 2 : 0);
+    batch_launch(b, (b->cur + s) & 1, mode);
+  }
+  hipLaunchKernelGGL(k_domain_finalize, dim3((b->ndom + 255) / 256), dim3(256), 0, b->stream, (const DomainDev *)b->doms_d,
+                     b->ndom, (const double *)b->partials, b->ntiles, b->active_d, b->res_d, b->mass_d, b->conv_d,
+                     (const long long *)b->cycle_d, tol);
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, b->stream, b->cycle_d);
+  BHIPCHK(b, hipStreamEndCapture(b->stream, &g));
+  BHIPCHK(b, hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+  BHIPCHK(b, hipGraphDestroy(g));
+  b->graph_interval = interval;
+  b->graph_tol = tol;
+  return LBM_OK;
+}
+
+int lbm_batch_run(lbm_batch *b, int32_t interval, double tol, int64_t max_steps, lbm_domain_result *results, double *ms) {
+  if (!b || interval < 2 || interval % 2 || max_steps < 0) return bfail(b, LBM_EINVAL, "lbm_batch_run: interval must be even >= 2");
+  if (!b->graph || b->graph_interval != interval || b->graph_tol != tol) {
+    const int rc = batch_capture(b, interval, tol);
+    if (rc) return rc;
+  }
+  std::vector<int> active(b->ndom);
+  hipEvent_t e0, e1;
+  BHIPCHK(b, hipEventCreate(&e0));
+  BHIPCHK(b, hipEventCreate(&e1));
+  BHIPCHK(b, hipEventRecord(e0, b->stream));
+  long long done = 0;
+  int tiled_active = 0;
+  for (int d = 0; d < b->ndom; ++d) tiled_active += b->active_h.empty() ? 1 : b->active_h[d];
+  while (done + interval <= max_steps) {
+    BHIPCHK(b, hipGraphLaunch(b->graph, b->stream));
+    done += interval;
+    BHIPCHK(b, hipMemcpyAsync(active.data(), b->active_d, b->ndom * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    int nact = 0;
+    for (int v : active) nact += v != 0;
+    if (!nact) break;
+    // drop the tiles of retired domains once a quarter of them is idle
+    if (nact * 4 <= tiled_active * 3) {
+      b->active_h = active;
+      int rc = batch_build_tiles(b, b->tx);
+      if (rc) return rc;
+      rc = batch_capture(b, interval, tol);
+      if (rc) return rc;
+      tiled_active = nact;
+    }
+  }
+  BHIPCHK(b, hipEventRecord(e1, b->stream));
+  BHIPCHK(b, hipEventSynchronize(e1));
+  float t = 0;
+  BHIPCHK(b, hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (results) {
+    std::vector<double> res(b->ndom), mass(b->ndom);
+    std::vector<long long> conv(b->ndom);
+    long long cyc = 0;
+    BHIPCHK(b, hipMemcpy(res.data(), b->res_d, b->ndom * sizeof(double), hipMemcpyDeviceToHost));
+    BHIPCHK(b, hipMemcpy(mass.data(), b->mass_d, b->ndom * sizeof(double), hipMemcpyDeviceToHost));
+    BHIPCHK(b, hipMemcpy(conv.data(), b->conv_d, b->ndom * sizeof(long long), hipMemcpyDeviceToHost));
+    BHIPCHK(b, hipMemcpy(&cyc, b->cycle_d, sizeof(long long), hipMemcpyDeviceToHost));
+    for (int d = 0; d < b->ndom; ++d) {
+      results[d].steps = conv[d] >= 0 ? (conv[d] + 1) * (long long)interval : conv[d];
+      results[d].res_rel = res[d];
+      results[d].mass = mass[d];
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_batch_get_macros(lbm_batch *b, int32_t dom, double *rho, double *u) {
+  if (!b || dom < 0 || dom >= b->ndom || !rho || !u) return bfail(b, LBM_EINVAL, "lbm_batch_get_macros: bad argument");
+  const DomainDev &D = b->doms[dom];
+  const long long n = (long long)D.nx * D.ny * D.nz;
+  double *rd = nullptr, *ud = nullptr;
+  BHIPCHK(b, hipMalloc(&rd, n * sizeof(double)));
+  BHIPCHK(b, hipMalloc(&ud, n * b->dim * sizeof(double)));
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.src = static_cast<char *>(b->buf[b->cur]) + D.pop_off * b->esize;
+  a.flags = b->flags + D.cell_off;
+  a.plane = D.plane;
+  a.nx = D.nx; a.ny = D.ny; a.nz = D.nz; a.px = D.px;
+  dim3 g((D.nx + 63) / 64, (D.ny + 3) / 4, D.nz), bl(64, 4, 1);
+  dispatch_qs<MacroL>(b->Q, b->storage, a, g, bl, b->stream, rd, ud);
+  BHIPCHK(b, hipGetLastError());
+  BHIPCHK(b, hipMemcpyAsync(rho, rd, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIPCHK(b, hipMemcpyAsync(u, ud, n * b->dim * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  BHIPCHK(b, hipFree(rd));
+  BHIPCHK(b, hipFree(ud));
+  return LBM_OK;
+}
+
+int lbm_batch_get_populations(lbm_batch *b, int32_t dom, double *f) {
+  if (!b || dom < 0 || dom >= b->ndom || !f) return bfail(b, LBM_EINVAL, "lbm_batch_get_populations: bad argument");
+  const DomainDev &D = b->doms[dom];
+  const long long n = (long long)D.nx * D.ny * D.nz;
+  double *fd = nullptr;
+  BHIPCHK(b, hipMalloc(&fd, n * b->Q * sizeof(double)));
+  KArgs a;
+  memset(&a, 0, sizeof(a));
+  a.src = static_cast<char *>(b->buf[b->cur]) + D.pop_off * b->esize;
+  a.plane = D.plane;
+  a.nx = D.nx; a.ny = D.ny; a.nz = D.nz; a.px = D.px;
+  dim3 g((D.nx + 63) / 64, (D.ny + 3) / 4, D.nz), bl(64, 4, 1);
+  dispatch_qs<GetL>(b->Q, b->storage, a, g, bl, b->stream, fd);
+  BHIPCHK(b, hipGetLastError());
+  BHIPCHK(b, hipMemcpyAsync(f, fd, n * b->Q * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  BHIPCHK(b, hipFree(fd));
+  return LBM_OK;
+}
+
+int lbm_batch_info(const lbm_batch *b, int64_t *ncell, int64_t *ntiles, int32_t *tile_x, int32_t *blocks) {
+  if (!b) return bfail(nullptr, LBM_EINVAL, "lbm_batch_info: null");
+  if (ncell) *ncell = b->compact_cells;
+  if (ntiles) *ntiles = b->ntiles;
+  if (tile_x) *tile_x = b->tx;
+  if (blocks) *blocks = b->blocks;
+  return LBM_OK;
+}
+
+const char *lbm_batch_last_error(const lbm_batch *b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+int lbm_batch_destroy(lbm_batch *b) {
+  if (!b) return LBM_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (b->buf[k]) (void)hipFree(b->buf[k]);
+    if (b->snap[k]) (void)hipFree(b->snap[k]);
+  }
+  void *ptrs[] = {b->flags, b->doms_d, b->tiles_d, b->active_d, b->res_d, b->mass_d, b->partials, b->conv_d, b->cycle_d};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return LBM_OK;
+}
+
+}  // extern "C"

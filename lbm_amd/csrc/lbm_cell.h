@@ -1,0 +1,387 @@
+// Device building blocks of the lattice-Boltzmann step: reductions, moments,
+// BGK collision in shifted variables, Zou-He faces, and the fused per-cell
+// pull update used by every step kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_lattice.h"
+
+namespace lbm {
+
+// cell index type: 64-bit, or 32-bit unsigned when plane * sizeof(ST) < 2^32
+template <bool W64>
+using Idx = typename std::conditional<W64, long long, unsigned>::type;
+
+constexpr int NSUM = 12;  // mass, mom[3], absdiff[4], diff_sq, norm_sq, nonfinite, fluid_cells
+
+struct FaceDev {
+  int kind;
+  int prio;
+  double u[3];
+  double rho;
+};
+
+struct KArgs {
+  const void *src;
+  void *dst;
+  const uint8_t *flags;  // padded layout, may be null
+  long long plane;       // elements between population planes
+  int nx, ny, nz, px;
+  int per[3];
+  FaceDev face[6];
+  double omega;
+  double F[3];
+  void *snap_in;
+  void *snap_out;
+  double *partials;
+};
+
+template <typename C>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using T = float4;
+};
+template <>
+struct Vec4<double> {
+  using T = double4;
+};
+
+// --------------------------------------------------------------------------
+// wave64 + block reduction of NSUM doubles; block result to partials[block]
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NT>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NSUM], double *partials, long long b, long long nb) {
+  __shared__ double red[NT / 64][NSUM];
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int lane = tid & 63, wid = tid >> 6;
+#pragma unroll
+  for (int k = 0; k < NSUM; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) red[wid][k] = v[k];
+  }
+  __syncthreads();
+  const int nw = (blockDim.x * blockDim.y) >> 6;
+  if (tid < NSUM) {
+    double s = 0;
+    for (int w2 = 0; w2 < nw; ++w2) s += red[w2][tid];
+    // structure of arrays: partials[k * nb + b] (coalesced second stage)
+    partials[(long long)tid * nb + b] = s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Collision in shifted variables.  h_i = f_i - w_i, rho = 1 + sum h,
+// rho*u = sum c h (+F/2, Guo), heq_i = w_i (drho + rho (3cu + 4.5cu^2 - 1.5u^2)),
+// h' = h + omega (heq - h) + (1 - omega/2) w_i [3(c_i - u).F + 9 (c_i.u)(c_i.F)].
+// Same algebra as (1-w) fold + w feq of src/lbm/solver.cpp:601-613 with
+// eq::defaultEq (src/lbm/equilibrium_func.h:52-54) after subtracting w_i.
+template <int Q, typename C>
+struct Moments {
+  C drho, rho, ux, uy, uz;
+};
+
+template <int Q, typename C, typename R>
+__device__ __forceinline__ Moments<Q, C> moments(const R (&h)[Q], C Fx, C Fy, C Fz) {
+  using L = Lat<Q>;
+  C drho = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) drho += C(h[j]);
+#pragma unroll
+  for (int j = 0; j < Q - 1; ++j) {
+    if (L::cx[j] == 1) jx += C(h[j]);
+    if (L::cx[j] == -1) jx -= C(h[j]);
+    if (L::cy[j] == 1) jy += C(h[j]);
+    if (L::cy[j] == -1) jy -= C(h[j]);
+    if (L::cz[j] == 1) jz += C(h[j]);
+    if (L::cz[j] == -1) jz -= C(h[j]);
+  }
+  Moments<Q, C> m;
+  m.drho = drho;
+  m.rho = C(1) + drho;
+  const C inv = C(1) / m.rho;
+  m.ux = (jx + C(0.5) * Fx) * inv;
+  m.uy = (jy + C(0.5) * Fy) * inv;
+  m.uz = (jz + C(0.5) * Fz) * inv;
+  return m;
+}
+
+// Population j of the cell at index `cell` of a plane-major buffer.  With
+// W64 false the per-plane byte offset is a 32-bit unsigned value added to the
+// plane's base address, which is uniform over the wave: the load or store
+// then takes one 32-bit VGPR offset plus an SGPR base instead of a 64-bit
+// VGPR address per access (the host selects it when plane * sizeof(ST) fits
+// in 32 bits).
+template <typename ST, bool W64>
+__device__ __forceinline__ const ST *pop_ptr(const ST *base, long long plane, int j, Idx<W64> cell) {
+  if (W64) return base + (long long)j * plane + (long long)cell;
+  return reinterpret_cast<const ST *>(reinterpret_cast<const char *>(base + (long long)j * plane) +
+                                      (unsigned)((unsigned)cell * (unsigned)sizeof(ST)));
+}
+template <typename ST, bool W64>
+__device__ __forceinline__ ST *pop_ptr(ST *base, long long plane, int j, Idx<W64> cell) {
+  return const_cast<ST *>(pop_ptr<ST, W64>(static_cast<const ST *>(base), plane, j, cell));
+}
+
+template <int Q, typename ST, bool NT, bool W64 = true>
+__device__ __forceinline__ void collide_store(const typename Store<ST>::R (&h)[Q], const Moments<Q, typename Store<ST>::C> &m,
+                                              typename Store<ST>::C omega, typename Store<ST>::C Fx,
+                                              typename Store<ST>::C Fy, typename Store<ST>::C Fz, bool force,
+                                              ST *dst, long long plane, Idx<W64> cell) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  const C usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+  const C uF = m.ux * Fx + m.uy * Fy + m.uz * Fz;
+  const C fpre = C(1) - C(0.5) * omega;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    C cu = 0, cF = 0;
+    if (L::cx[j] == 1) { cu += m.ux; cF += Fx; }
+    if (L::cx[j] == -1) { cu -= m.ux; cF -= Fx; }
+    if (L::cy[j] == 1) { cu += m.uy; cF += Fy; }
+    if (L::cy[j] == -1) { cu -= m.uy; cF -= Fy; }
+    if (L::cz[j] == 1) { cu += m.uz; cF += Fz; }
+    if (L::cz[j] == -1) { cu -= m.uz; cF -= Fz; }
+    const C wj = C(L::w[j]);
+    const C heq = wj * (m.drho + m.rho * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+    const C hj = C(h[j]);
+    C out = hj + omega * (heq - hj);
+    if (force) out += fpre * wj * (C(3) * (cF - uF) + C(9) * cu * cF);
+    ST *p = pop_ptr<ST, W64>(dst, plane, j, cell);
+    if (NT) Store<ST>::st_nt(p, out);
+    else Store<ST>::st(p, out);
+  }
+}
+
+// Zou-He face (Hecht & Harting 2010 form) on shifted populations; axis and
+// outward sign of the face normal; velocity face prescribes u, pressure
+// face prescribes rho with zero tangential velocity.
+template <int Q, typename C>
+__device__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C u2, C rho_p) {
+  using L = Lat<Q>;
+  auto cax = [](int j, int a) { return a == 0 ? L::cx[j] : (a == 1 ? L::cy[j] : L::cz[j]); };
+  C s0 = 0, sout = 0;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int cn = cax(j, axis) * sign;
+    if (cn == 0) s0 += h[j];
+    else if (cn > 0) sout += h[j];
+  }
+  C u[3] = {0, 0, 0};
+  C rho;
+  if (is_vel) {
+    u[0] = u0; u[1] = u1; u[2] = u2;
+    const C un = (axis == 0 ? u0 : (axis == 1 ? u1 : u2)) * C(sign);
+    rho = (C(1) + s0 + C(2) * sout) / (C(1) + un);
+  } else {
+    rho = rho_p;
+    const C un = (C(1) + s0 + C(2) * sout) / rho - C(1);
+    if (axis == 0) u[0] = un * C(sign);
+    if (axis == 1) u[1] = un * C(sign);
+    if (axis == 2) u[2] = un * C(sign);
+  }
+  C N[3] = {0, 0, 0};
+#pragma unroll
+  for (int t = 0; t < Lat<Q>::D; ++t) {
+    if (t == axis) continue;
+    C mt = 0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+      if (cax(j, axis) == 0) mt += C(cax(j, t)) * h[j];
+    N[t] = C(0.5) * mt - rho * u[t] / C(3);
+  }
+  C hn[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) hn[j] = h[j];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int cn = cax(j, axis) * sign;
+    if (!(cn < 0)) continue;
+    C cu = 0, corr = 0;
+#pragma unroll
+    for (int d = 0; d < Lat<Q>::D; ++d) cu += C(cax(j, d)) * u[d];
+#pragma unroll
+    for (int t = 0; t < Lat<Q>::D; ++t)
+      if (t != axis) corr += C(cax(j, t)) * N[t];
+    hn[j] = h[L::opp[j]] + C(6.0 * L::w[j]) * rho * cu - corr;
+  }
+#pragma unroll
+  for (int j = 0; j < Q; ++j) h[j] = hn[j];
+}
+
+// velocity u* of the stored post-collision state at a cell (used for the
+// anti-bounce-back extrapolation: collision conserves rho and rho*u, so the
+// moments of S equal those of the pre-collision populations plus F).
+template <int Q, typename ST>
+__device__ void cell_velocity(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
+                              typename Store<ST>::C (&u)[3]) {
+  using C = typename Store<ST>::C;
+  typename Store<ST>::R h[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) h[j] = Store<ST>::ld(src + j * plane + cell);
+  const Moments<Q, C> m = moments<Q, C>(h, C(-Fx), C(-Fy), C(-Fz));  // (m_S - F/2)/rho
+  u[0] = m.ux; u[1] = m.uy; u[2] = m.uz;
+}
+
+// --------------------------------------------------------------------------
+// The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
+// 2: reduce against the snapshot written by the previous step; 3: both
+// (reduce against snap_in, write snap_out).
+// Per-cell update shared by the single-domain and batched kernels.  G is a
+// view of one structured domain with members src, dst, flags, plane, nx, ny,
+// nz, px, per[3], face[6], omega, F[3], snap_in, snap_out (the kernel
+// arguments for one domain, or a domain of a batch).  red must be zeroed by
+// the caller; MODE >= 2 accumulates this cell's sums into it.
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, typename G>
+__device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, double (&red)[NSUM]) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  using I = Idx<W64>;
+  const ST *__restrict__ src = static_cast<const ST *>(a.src);
+  ST *__restrict__ dst = static_cast<ST *>(a.dst);
+  const long long plane = a.plane;
+  const I cell = I(x) + I(a.px) * (I(y) + I(a.ny) * I(z));
+  bool active = x < a.nx && y < a.ny;
+  if (FLAGS && active) active = a.flags[cell] == LBM_CELL_FLUID;
+
+  if (active) {
+    // neighbour coordinates per axis with periodic wrap; v* = exists
+    int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
+    bool vxm = true, vxp = true, vym = true, vyp = true, vzm = true, vzp = true;
+    if (x == 0) { if (a.per[0]) xm = a.nx - 1; else vxm = false; }
+    if (x == a.nx - 1) { if (a.per[0]) xp = 0; else vxp = false; }
+    if (y == 0) { if (a.per[1]) ym = a.ny - 1; else vym = false; }
+    if (y == a.ny - 1) { if (a.per[1]) yp = 0; else vyp = false; }
+    if (Q == 19) {
+      if (z == 0) { if (a.per[2]) zm = a.nz - 1; else vzm = false; }
+      if (z == a.nz - 1) { if (a.per[2]) zp = 0; else vzp = false; }
+    }
+    using R = typename Store<ST>::R;
+    R h[Q];
+    unsigned missing = 0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+      const int sx = cx > 0 ? xm : (cx < 0 ? xp : x);
+      const int sy = cy > 0 ? ym : (cy < 0 ? yp : y);
+      const int sz = cz > 0 ? zm : (cz < 0 ? zp : z);
+      bool valid = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true)) &&
+                   (cz > 0 ? vzm : (cz < 0 ? vzp : true));
+      const I s = I(sx) + I(a.px) * (I(sy) + I(a.ny) * I(sz));
+      // A population streams iff its upstream cell is fluid.  This is the
+      // symmetric rule; the reference composes diagonal neighbours from main
+      // ones (src/cartesiangrid.h:466-486), which next to staircase obstacles
+      // makes the neighbour relation one-sided and leaves some populations
+      // unwritten for a step.
+      if (FLAGS && valid && j != Q - 1) valid = a.flags[s] == LBM_CELL_FLUID;
+      // a missing upstream cell loads the cell's own entry of plane j (in
+      // bounds, same cache lines as the neighbours' loads) and is replaced
+      // below, so every load of the loop keeps the uniform base of plane j
+      const ST *p = pop_ptr<ST, W64>(src, plane, j, valid ? s : cell);
+      h[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
+      if (!valid) missing |= 1u << j;
+    }
+
+    const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
+    const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
+
+    if (missing) {
+      // bounce-back source S(x, opp j) of every missing population
+#pragma unroll
+      for (int j = 0; j < Q - 1; ++j)
+        if ((missing >> j) & 1u) h[j] = Store<ST>::ld(pop_ptr<ST, W64>(src, plane, L::opp[j], cell));
+      // boundary substitution for populations whose upstream cell is missing
+      C uext[3] = {0, 0, 0};
+      int abb_face = -1;
+#pragma unroll
+      for (int j = 0; j < Q - 1; ++j) {
+        if (!((missing >> j) & 1u)) continue;
+        const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+        const bool ix = (cx > 0 && !vxm) || (cx < 0 && !vxp);
+        const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
+        const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
+        int f = -1;
+        // crossed faces: the higher priority wins, ties to the lower axis
+        if (ix) f = cx > 0 ? 0 : 1;
+        if (iy) { const int g = cy > 0 ? 2 : 3; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
+        if (iz) { const int g = cz > 0 ? 4 : 5; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
+        if (f < 0) continue;  // solid obstacle: plain bounce-back, h[j] already holds S(x, opp j)
+        const FaceDev &fd = a.face[f];
+        const C wj = C(L::w[j]);
+        if (fd.kind == LBM_FACE_WALL) {
+          // Ladd moving wall, rho0 = 1 (bnd_dirichlet.h:109-116)
+          const C cu = C(cx) * C(fd.u[0]) + C(cy) * C(fd.u[1]) + C(cz) * C(fd.u[2]);
+          h[j] = R(C(h[j]) + C(6) * wj * cu);
+        } else if (fd.kind == LBM_FACE_PRESSURE_ABB) {
+          if (abb_face != f) {
+            // u extrapolated from the two inward cells (bnd_pressure.h:70-81)
+            const int ax = f >> 1, out = (f & 1) ? 1 : -1;
+            const int dx = ax == 0 ? -out : 0, dy = ax == 1 ? -out : 0, dz = ax == 2 ? -out : 0;
+            const long long n1 = (long long)(x + dx) + (long long)a.px * ((long long)(y + dy) + (long long)a.ny * (z + dz));
+            const long long n2 = (long long)(x + 2 * dx) + (long long)a.px * ((long long)(y + 2 * dy) + (long long)a.ny * (z + 2 * dz));
+            C u1[3], u2[3];
+            cell_velocity<Q, ST>(src, plane, n1, a.F[0], a.F[1], a.F[2], u1);
+            cell_velocity<Q, ST>(src, plane, n2, a.F[0], a.F[1], a.F[2], u2);
+            for (int d = 0; d < 3; ++d) uext[d] = C(1.5) * u1[d] - C(0.5) * u2[d];
+            abb_face = f;
+          }
+          const C cu = C(cx) * uext[0] + C(cy) * uext[1] + C(cz) * uext[2];
+          const C vsq = uext[0] * uext[0] + uext[1] * uext[1] + uext[2] * uext[2];
+          // f = -f_post + 2 symmEq(p, u) in shifted form
+          h[j] = R(-C(h[j]) + wj * (C(2) * C(fd.rho) * (C(1) + C(4.5) * cu * cu - C(1.5) * vsq) - C(2)));
+        }
+        // Zou-He faces: overwritten below
+      }
+      // Zou-He on the first Zou-He face the cell lies on
+#pragma unroll 1
+      for (int f = 0; f < 2 * L::D; ++f) {
+        const int k = a.face[f].kind;
+        if (k != LBM_FACE_VELOCITY_ZH && k != LBM_FACE_PRESSURE_ZH) continue;
+        const int ax = f >> 1;
+        const int coord = ax == 0 ? x : (ax == 1 ? y : z);
+        const int n = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+        if (coord != ((f & 1) ? n - 1 : 0)) continue;
+        zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
+                     R(a.face[f].u[2]), R(a.face[f].rho));
+        break;
+      }
+    }
+
+    const Moments<Q, C> m = moments<Q, C>(h, Fx, Fy, Fz);
+    using V4 = typename Vec4<C>::T;
+    if (MODE == 1 || MODE == 3) {
+      V4 v;
+      v.x = m.ux; v.y = m.uy; v.z = m.uz; v.w = m.rho;
+      static_cast<V4 *>(a.snap_out)[cell] = v;
+    }
+    if (MODE >= 2) {
+      const V4 o = static_cast<const V4 *>(a.snap_in)[cell];
+      const double ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
+      const double dux = ux - (double)o.x, duy = uy - (double)o.y, duz = uz - (double)o.z, drh = rho - (double)o.w;
+      red[0] = rho;
+      red[1] = rho * ux;
+      red[2] = rho * uy;
+      red[3] = rho * uz;
+      red[4] = fabs(dux);
+      red[5] = fabs(duy);
+      red[6] = fabs(duz);
+      red[7] = fabs(drh);
+      red[8] = dux * dux + duy * duy + duz * duz;
+      red[9] = ux * ux + uy * uy + uz * uz;
+      red[10] = (isfinite(rho) && isfinite(ux) && isfinite(uy) && isfinite(uz)) ? 0.0 : 1.0;
+      red[11] = 1.0;
+    }
+    collide_store<Q, ST, NT, W64>(h, m, C(a.omega), Fx, Fy, Fz, force, dst, plane, cell);
+  }
+}
+
+}  // namespace lbm

@@ -17,8 +17,9 @@
 // written once: 2*Q loads+stores per cell, no neighbour table on device.
 //
 // Populations are stored shifted, h = f - w (density deviation form), which
-// keeps FP32 and 16-bit storage well conditioned; collisions run in FP32
-// (FP64 for FP64 storage).  Checked steps reduce mass, momentum and the
+// keeps FP32 and 16-bit storage well conditioned; moments, equilibrium and
+// collision run in FP64 for FP32 and FP64 storage and in FP32 for 16-bit
+// storage.  Checked steps reduce mass, momentum and the
 // one-step change of (u, rho) in FP64 with a fixed-order two-stage tree.
 #include <hip/hip_runtime.h>
 
@@ -30,75 +31,10 @@
 #include <vector>
 
 #include "../../include/lbm_hip.h"
-#include "lbm_lattice.h"
+#include "lbm_cell.h"
+#include "lbm_launch.h"
 
 namespace lbm {
-
-constexpr int NSUM = 12;  // mass, mom[3], absdiff[4], diff_sq, norm_sq, nonfinite, fluid_cells
-
-struct FaceDev {
-  int kind;
-  int prio;
-  double u[3];
-  double rho;
-};
-
-struct KArgs {
-  const void *src;
-  void *dst;
-  const uint8_t *flags;  // padded layout, may be null
-  long long plane;       // elements between population planes
-  int nx, ny, nz, px;
-  int per[3];
-  FaceDev face[6];
-  double omega;
-  double F[3];
-  void *snap_in;
-  void *snap_out;
-  double *partials;
-};
-
-template <typename C>
-struct Vec4;
-template <>
-struct Vec4<float> {
-  using T = float4;
-};
-template <>
-struct Vec4<double> {
-  using T = double4;
-};
-
-// --------------------------------------------------------------------------
-// wave64 + block reduction of NSUM doubles; block result to partials[block]
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-template <int NT>
-__device__ __forceinline__ void block_reduce_store(double (&v)[NSUM], double *partials) {
-  __shared__ double red[NT / 64][NSUM];
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int lane = tid & 63, wid = tid >> 6;
-#pragma unroll
-  for (int k = 0; k < NSUM; ++k) v[k] = wave_sum(v[k]);
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < NSUM; ++k) red[wid][k] = v[k];
-  }
-  __syncthreads();
-  const int nw = (blockDim.x * blockDim.y) >> 6;
-  if (tid < NSUM) {
-    double s = 0;
-    for (int w2 = 0; w2 < nw; ++w2) s += red[w2][tid];
-    // structure of arrays: partials[k * nblocks + block] (coalesced second stage)
-    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
-    const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-    partials[(long long)tid * nb + b] = s;
-  }
-}
 
 // Fixed-order second stage: block k reduces sum k; thread t sums blocks
 // t, t+1024, ... in order, then a fixed pairwise tree.
@@ -115,296 +51,6 @@ __global__ void __launch_bounds__(1024) k_finalize(const double *partials, long 
     __syncthreads();
   }
   if (threadIdx.x == 0) out[k] = s[0];
-}
-
-// --------------------------------------------------------------------------
-// Collision in shifted variables.  h_i = f_i - w_i, rho = 1 + sum h,
-// rho*u = sum c h (+F/2, Guo), heq_i = w_i (drho + rho (3cu + 4.5cu^2 - 1.5u^2)),
-// h' = h + omega (heq - h) + (1 - omega/2) w_i [3(c_i - u).F + 9 (c_i.u)(c_i.F)].
-// Same algebra as (1-w) fold + w feq of src/lbm/solver.cpp:601-613 with
-// eq::defaultEq (src/lbm/equilibrium_func.h:52-54) after subtracting w_i.
-template <int Q, typename C>
-struct Moments {
-  C drho, rho, ux, uy, uz;
-};
-
-template <int Q, typename C>
-__device__ __forceinline__ Moments<Q, C> moments(const C (&h)[Q], C Fx, C Fy, C Fz) {
-  using L = Lat<Q>;
-  C drho = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int j = 0; j < Q; ++j) drho += h[j];
-#pragma unroll
-  for (int j = 0; j < Q - 1; ++j) {
-    if (L::cx[j] == 1) jx += h[j];
-    if (L::cx[j] == -1) jx -= h[j];
-    if (L::cy[j] == 1) jy += h[j];
-    if (L::cy[j] == -1) jy -= h[j];
-    if (L::cz[j] == 1) jz += h[j];
-    if (L::cz[j] == -1) jz -= h[j];
-  }
-  Moments<Q, C> m;
-  m.drho = drho;
-  m.rho = C(1) + drho;
-  const C inv = C(1) / m.rho;
-  m.ux = (jx + C(0.5) * Fx) * inv;
-  m.uy = (jy + C(0.5) * Fy) * inv;
-  m.uz = (jz + C(0.5) * Fz) * inv;
-  return m;
-}
-
-template <int Q, typename ST, bool NT>
-__device__ __forceinline__ void collide_store(typename Store<ST>::C (&h)[Q], const Moments<Q, typename Store<ST>::C> &m,
-                                              typename Store<ST>::C omega, typename Store<ST>::C Fx,
-                                              typename Store<ST>::C Fy, typename Store<ST>::C Fz, bool force,
-                                              ST *dst, long long plane, long long cell) {
-  using L = Lat<Q>;
-  using C = typename Store<ST>::C;
-  const C usq = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
-  const C uF = m.ux * Fx + m.uy * Fy + m.uz * Fz;
-  const C fpre = C(1) - C(0.5) * omega;
-#pragma unroll
-  for (int j = 0; j < Q; ++j) {
-    C cu = 0, cF = 0;
-    if (L::cx[j] == 1) { cu += m.ux; cF += Fx; }
-    if (L::cx[j] == -1) { cu -= m.ux; cF -= Fx; }
-    if (L::cy[j] == 1) { cu += m.uy; cF += Fy; }
-    if (L::cy[j] == -1) { cu -= m.uy; cF -= Fy; }
-    if (L::cz[j] == 1) { cu += m.uz; cF += Fz; }
-    if (L::cz[j] == -1) { cu -= m.uz; cF -= Fz; }
-    const C wj = C(L::w[j]);
-    const C heq = wj * (m.drho + m.rho * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
-    C out = h[j] + omega * (heq - h[j]);
-    if (force) out += fpre * wj * (C(3) * (cF - uF) + C(9) * cu * cF);
-    if (NT) Store<ST>::st_nt(dst + j * plane + cell, out);
-    else Store<ST>::st(dst + j * plane + cell, out);
-  }
-}
-
-// Zou-He face (Hecht & Harting 2010 form) on shifted populations; axis and
-// outward sign of the face normal; velocity face prescribes u, pressure
-// face prescribes rho with zero tangential velocity.
-template <int Q, typename C>
-__device__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C u2, C rho_p) {
-  using L = Lat<Q>;
-  auto cax = [](int j, int a) { return a == 0 ? L::cx[j] : (a == 1 ? L::cy[j] : L::cz[j]); };
-  C s0 = 0, sout = 0;
-#pragma unroll
-  for (int j = 0; j < Q; ++j) {
-    const int cn = cax(j, axis) * sign;
-    if (cn == 0) s0 += h[j];
-    else if (cn > 0) sout += h[j];
-  }
-  C u[3] = {0, 0, 0};
-  C rho;
-  if (is_vel) {
-    u[0] = u0; u[1] = u1; u[2] = u2;
-    const C un = (axis == 0 ? u0 : (axis == 1 ? u1 : u2)) * C(sign);
-    rho = (C(1) + s0 + C(2) * sout) / (C(1) + un);
-  } else {
-    rho = rho_p;
-    const C un = (C(1) + s0 + C(2) * sout) / rho - C(1);
-    if (axis == 0) u[0] = un * C(sign);
-    if (axis == 1) u[1] = un * C(sign);
-    if (axis == 2) u[2] = un * C(sign);
-  }
-  C N[3] = {0, 0, 0};
-#pragma unroll
-  for (int t = 0; t < Lat<Q>::D; ++t) {
-    if (t == axis) continue;
-    C mt = 0;
-#pragma unroll
-    for (int j = 0; j < Q; ++j)
-      if (cax(j, axis) == 0) mt += C(cax(j, t)) * h[j];
-    N[t] = C(0.5) * mt - rho * u[t] / C(3);
-  }
-  C hn[Q];
-#pragma unroll
-  for (int j = 0; j < Q; ++j) hn[j] = h[j];
-#pragma unroll
-  for (int j = 0; j < Q; ++j) {
-    const int cn = cax(j, axis) * sign;
-    if (!(cn < 0)) continue;
-    C cu = 0, corr = 0;
-#pragma unroll
-    for (int d = 0; d < Lat<Q>::D; ++d) cu += C(cax(j, d)) * u[d];
-#pragma unroll
-    for (int t = 0; t < Lat<Q>::D; ++t)
-      if (t != axis) corr += C(cax(j, t)) * N[t];
-    hn[j] = h[L::opp[j]] + C(6.0 * L::w[j]) * rho * cu - corr;
-  }
-#pragma unroll
-  for (int j = 0; j < Q; ++j) h[j] = hn[j];
-}
-
-// velocity u* of the stored post-collision state at a cell (used for the
-// anti-bounce-back extrapolation: collision conserves rho and rho*u, so the
-// moments of S equal those of the pre-collision populations plus F).
-template <int Q, typename ST>
-__device__ void cell_velocity(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
-                              typename Store<ST>::C (&u)[3]) {
-  using C = typename Store<ST>::C;
-  C h[Q];
-#pragma unroll
-  for (int j = 0; j < Q; ++j) h[j] = Store<ST>::ld(src + j * plane + cell);
-  const Moments<Q, C> m = moments<Q, C>(h, C(-Fx), C(-Fy), C(-Fz));  // (m_S - F/2)/rho
-  u[0] = m.ux; u[1] = m.uy; u[2] = m.uz;
-}
-
-// --------------------------------------------------------------------------
-// The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
-// 2: reduce against the snapshot written by the previous step; 3: both
-// (reduce against snap_in, write snap_out).
-#ifndef LBM_STEP_WAVES_PER_EU
-#define LBM_STEP_WAVES_PER_EU 0
-#endif
-#if LBM_STEP_WAVES_PER_EU > 0
-#define LBM_STEP_ATTR __attribute__((amdgpu_waves_per_eu(LBM_STEP_WAVES_PER_EU)))
-#else
-#define LBM_STEP_ATTR
-#endif
-template <int Q, typename ST, bool FLAGS, int MODE, bool NT>
-__global__ void __launch_bounds__(256) LBM_STEP_ATTR k_step(const KArgs a) {
-  using L = Lat<Q>;
-  using C = typename Store<ST>::C;
-  const ST *__restrict__ src = static_cast<const ST *>(a.src);
-  ST *__restrict__ dst = static_cast<ST *>(a.dst);
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
-  const long long plane = a.plane;
-  const long long cell = (long long)x + (long long)a.px * ((long long)y + (long long)a.ny * z);
-  bool active = x < a.nx && y < a.ny;
-  if (FLAGS && active) active = a.flags[cell] == LBM_CELL_FLUID;
-
-  double red[NSUM];
-  if (MODE >= 2) {
-#pragma unroll
-    for (int k = 0; k < NSUM; ++k) red[k] = 0;
-  }
-
-  if (active) {
-    // neighbour coordinates per axis with periodic wrap; v* = exists
-    int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
-    bool vxm = true, vxp = true, vym = true, vyp = true, vzm = true, vzp = true;
-    if (x == 0) { if (a.per[0]) xm = a.nx - 1; else vxm = false; }
-    if (x == a.nx - 1) { if (a.per[0]) xp = 0; else vxp = false; }
-    if (y == 0) { if (a.per[1]) ym = a.ny - 1; else vym = false; }
-    if (y == a.ny - 1) { if (a.per[1]) yp = 0; else vyp = false; }
-    if (Q == 19) {
-      if (z == 0) { if (a.per[2]) zm = a.nz - 1; else vzm = false; }
-      if (z == a.nz - 1) { if (a.per[2]) zp = 0; else vzp = false; }
-    }
-    C h[Q];
-    unsigned missing = 0;
-#pragma unroll
-    for (int j = 0; j < Q; ++j) {
-      const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
-      const int sx = cx > 0 ? xm : (cx < 0 ? xp : x);
-      const int sy = cy > 0 ? ym : (cy < 0 ? yp : y);
-      const int sz = cz > 0 ? zm : (cz < 0 ? zp : z);
-      bool valid = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true)) &&
-                   (cz > 0 ? vzm : (cz < 0 ? vzp : true));
-      const long long s = (long long)sx + (long long)a.px * ((long long)sy + (long long)a.ny * sz);
-      // A population streams iff its upstream cell is fluid.  This is the
-      // symmetric rule; the reference composes diagonal neighbours from main
-      // ones (src/cartesiangrid.h:466-486), which next to staircase obstacles
-      // makes the neighbour relation one-sided and leaves some populations
-      // unwritten for a step.
-      if (FLAGS && valid && j != Q - 1) valid = a.flags[s] == LBM_CELL_FLUID;
-      const long long addr = valid ? (long long)j * plane + s : (long long)L::opp[j] * plane + cell;
-      h[j] = NT ? Store<ST>::ld_nt(src + addr) : Store<ST>::ld(src + addr);
-      if (!valid) missing |= 1u << j;
-    }
-
-    const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
-    const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
-
-    if (missing) {
-      // boundary substitution for populations whose upstream cell is missing
-      C uext[3] = {0, 0, 0};
-      int abb_face = -1;
-#pragma unroll
-      for (int j = 0; j < Q - 1; ++j) {
-        if (!((missing >> j) & 1u)) continue;
-        const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
-        const bool ix = (cx > 0 && !vxm) || (cx < 0 && !vxp);
-        const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
-        const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
-        int f = -1;
-        // crossed faces: the higher priority wins, ties to the lower axis
-        if (ix) f = cx > 0 ? 0 : 1;
-        if (iy) { const int g = cy > 0 ? 2 : 3; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
-        if (iz) { const int g = cz > 0 ? 4 : 5; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
-        if (f < 0) continue;  // solid obstacle: plain bounce-back, h[j] already holds S(x, opp j)
-        const FaceDev &fd = a.face[f];
-        const C wj = C(L::w[j]);
-        if (fd.kind == LBM_FACE_WALL) {
-          // Ladd moving wall, rho0 = 1 (bnd_dirichlet.h:109-116)
-          const C cu = C(cx) * C(fd.u[0]) + C(cy) * C(fd.u[1]) + C(cz) * C(fd.u[2]);
-          h[j] += C(6) * wj * cu;
-        } else if (fd.kind == LBM_FACE_PRESSURE_ABB) {
-          if (abb_face != f) {
-            // u extrapolated from the two inward cells (bnd_pressure.h:70-81)
-            const int ax = f >> 1, out = (f & 1) ? 1 : -1;
-            const int dx = ax == 0 ? -out : 0, dy = ax == 1 ? -out : 0, dz = ax == 2 ? -out : 0;
-            const long long n1 = (long long)(x + dx) + (long long)a.px * ((long long)(y + dy) + (long long)a.ny * (z + dz));
-            const long long n2 = (long long)(x + 2 * dx) + (long long)a.px * ((long long)(y + 2 * dy) + (long long)a.ny * (z + 2 * dz));
-            C u1[3], u2[3];
-            cell_velocity<Q, ST>(src, plane, n1, a.F[0], a.F[1], a.F[2], u1);
-            cell_velocity<Q, ST>(src, plane, n2, a.F[0], a.F[1], a.F[2], u2);
-            for (int d = 0; d < 3; ++d) uext[d] = C(1.5) * u1[d] - C(0.5) * u2[d];
-            abb_face = f;
-          }
-          const C cu = C(cx) * uext[0] + C(cy) * uext[1] + C(cz) * uext[2];
-          const C vsq = uext[0] * uext[0] + uext[1] * uext[1] + uext[2] * uext[2];
-          // f = -f_post + 2 symmEq(p, u) in shifted form
-          h[j] = -h[j] + wj * (C(2) * C(fd.rho) * (C(1) + C(4.5) * cu * cu - C(1.5) * vsq) - C(2));
-        }
-        // Zou-He faces: overwritten below
-      }
-      // Zou-He on the first Zou-He face the cell lies on
-#pragma unroll 1
-      for (int f = 0; f < 2 * L::D; ++f) {
-        const int k = a.face[f].kind;
-        if (k != LBM_FACE_VELOCITY_ZH && k != LBM_FACE_PRESSURE_ZH) continue;
-        const int ax = f >> 1;
-        const int coord = ax == 0 ? x : (ax == 1 ? y : z);
-        const int n = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
-        if (coord != ((f & 1) ? n - 1 : 0)) continue;
-        zou_he<Q, C>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, C(a.face[f].u[0]), C(a.face[f].u[1]),
-                     C(a.face[f].u[2]), C(a.face[f].rho));
-        break;
-      }
-    }
-
-    const Moments<Q, C> m = moments<Q, C>(h, Fx, Fy, Fz);
-    using V4 = typename Vec4<C>::T;
-    if (MODE == 1 || MODE == 3) {
-      V4 v;
-      v.x = m.ux; v.y = m.uy; v.z = m.uz; v.w = m.rho;
-      static_cast<V4 *>(a.snap_out)[cell] = v;
-    }
-    if (MODE >= 2) {
-      const V4 o = static_cast<const V4 *>(a.snap_in)[cell];
-      const double ux = m.ux, uy = m.uy, uz = m.uz, rho = m.rho;
-      const double dux = ux - (double)o.x, duy = uy - (double)o.y, duz = uz - (double)o.z, drh = rho - (double)o.w;
-      red[0] = rho;
-      red[1] = rho * ux;
-      red[2] = rho * uy;
-      red[3] = rho * uz;
-      red[4] = fabs(dux);
-      red[5] = fabs(duy);
-      red[6] = fabs(duz);
-      red[7] = fabs(drh);
-      red[8] = dux * dux + duy * duy + duz * duz;
-      red[9] = ux * ux + uy * uy + uz * uz;
-      red[10] = (isfinite(rho) && isfinite(ux) && isfinite(uy) && isfinite(uz)) ? 0.0 : 1.0;
-      red[11] = 1.0;
-    }
-    collide_store<Q, ST, NT>(h, m, C(a.omega), Fx, Fy, Fz, force, dst, plane, cell);
-  }
-  if (MODE >= 2) block_reduce_store<256>(red, a.partials);
 }
 
 // Initial condition: h = feq(rho, u) - w, then the first collision, i.e. the
@@ -427,11 +73,11 @@ __global__ void __launch_bounds__(256) k_init(KArgs a, const double *rho, const 
   const C r = C(rho[cc]);
   const C u0 = C(u[cc * L::D + 0]), u1 = C(u[cc * L::D + 1]), u2 = L::D == 3 ? C(u[cc * L::D + 2]) : C(0);
   const C usq = u0 * u0 + u1 * u1 + u2 * u2;
-  C h[Q];
+  typename Store<ST>::R h[Q];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
     const C cu = C(L::cx[j]) * u0 + C(L::cy[j]) * u1 + C(L::cz[j]) * u2;
-    h[j] = C(L::w[j]) * ((r - C(1)) + r * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+    h[j] = typename Store<ST>::R(C(L::w[j]) * ((r - C(1)) + r * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq)));
   }
   const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
   const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
@@ -462,7 +108,7 @@ __global__ void __launch_bounds__(256) k_macros(KArgs a, double *rho, double *u)
   for (int k = 0; k < NSUM; ++k) red[k] = 0;
   if (active) {
     const ST *src = static_cast<const ST *>(a.src);
-    C h[Q];
+    typename Store<ST>::R h[Q];
 #pragma unroll
     for (int j = 0; j < Q; ++j) h[j] = Store<ST>::ld(src + j * a.plane + cell);
     const Moments<Q, C> m = moments<Q, C>(h, C(-a.F[0]), C(-a.F[1]), C(-a.F[2]));
@@ -481,7 +127,11 @@ __global__ void __launch_bounds__(256) k_macros(KArgs a, double *rho, double *u)
       red[11] = 1.0;
     }
   }
-  if (a.partials) block_reduce_store<256>(red, a.partials);
+  if (a.partials) {
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
+    const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    block_reduce_store<256>(red, a.partials, b, nb);
+  }
 }
 
 template <int Q, typename ST>
@@ -556,6 +206,7 @@ struct lbm_handle {
   double *sums_d = nullptr;
   hipStream_t stream = nullptr;
   int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
+  bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
   long long steps = 0;
   std::string err;
 };
@@ -621,26 +272,6 @@ static void dispatch_qs(int Q, int storage, Args... args) {
   }
 }
 
-template <int Q, typename ST>
-struct StepL {
-  static void run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt) {
-    if (flags) {
-      if (mode == 0) {
-        if (nt) hipLaunchKernelGGL((k_step<Q, ST, true, 0, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_step<Q, ST, true, 0, false>), g, b, 0, s, a);
-      } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, true, 1, false>), g, b, 0, s, a);
-      else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, true, 2, false>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_step<Q, ST, true, 3, false>), g, b, 0, s, a);
-    } else {
-      if (mode == 0) {
-        if (nt) hipLaunchKernelGGL((k_step<Q, ST, false, 0, true>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_step<Q, ST, false, 0, false>), g, b, 0, s, a);
-      } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, false, 1, false>), g, b, 0, s, a);
-      else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, false, 2, false>), g, b, 0, s, a);
-      else hipLaunchKernelGGL((k_step<Q, ST, false, 3, false>), g, b, 0, s, a);
-    }
-  }
-};
 template <int Q, typename ST>
 struct InitL {
   static void run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, const double *r, const double *u) {
@@ -712,6 +343,7 @@ int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out)
   long long pad = 0;
   if (const char *e = getenv("LBM_PLANE_PAD")) pad = atoll(e);
   h->plane = (long long)h->px * h->ny * h->nz + pad;
+  h->w64 = (unsigned long long)h->plane * (unsigned long long)es > 0xffffffffull;
   h->omega = p->omega;
   for (int i = 0; i < 3; ++i) h->F[i] = p->force[i];
   for (int f = 0; f < 6; ++f) { memset(&h->face[f], 0, sizeof(FaceDev)); h->face[f].kind = LBM_FACE_PERIODIC; h->face[f].rho = 1.0; }
@@ -821,8 +453,17 @@ int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
 
 static void fill_sums(const double *s, int dim, lbm_sums *out);
 
+// bytes of one (u, rho) snapshot entry: four values of the compute type
+static size_t snap_bytes(int storage) {
+  switch (storage) {
+    case LBM_STORE_F32: return sizeof(typename Vec4<typename Store<float>::C>::T);
+    case LBM_STORE_F64: return sizeof(double4);
+    default: return sizeof(float4);
+  }
+}
+
 static int ensure_snapshots(lbm_handle *h) {
-  const size_t cs = h->storage == LBM_STORE_F64 ? sizeof(double4) : sizeof(float4);
+  const size_t cs = snap_bytes(h->storage);
   for (int b = 0; b < 2; ++b)
     if (!h->snap[b]) HIPCHK(h, hipMalloc(&h->snap[b], (size_t)h->px * h->ny * h->nz * cs));
   return LBM_OK;
@@ -839,7 +480,7 @@ static int launch_step(lbm_handle *h, int mode) {
   }
   dim3 g, b;
   launch_dims(h, g, b);
-  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0);
+  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0, h->w64);
   HIPCHK(h, hipGetLastError());
   if (mode == 1 || mode == 3) h->snap_last ^= 1;
   h->cur ^= 1;
@@ -876,7 +517,8 @@ static void fill_sums(const double *s, int dim, lbm_sums *out) {
   out->abs_diff[dim] = s[7];
   out->diff_sq = s[8];
   out->norm_sq = s[9];
-  out->res_rel = s[9] > 0 ? sqrt(s[8] / s[9]) : (s[8] > 0 ? INFINITY : 0.0);
+  if (!std::isfinite(s[8]) || !std::isfinite(s[9])) out->res_rel = NAN;
+  else out->res_rel = s[9] > 0 ? sqrt(s[8] / s[9]) : (s[8] > 0 ? INFINITY : 0.0);
   out->nonfinite = s[10];
   out->fluid_cells = s[11];
 }
@@ -1003,6 +645,20 @@ int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nont
   return ensure_partials(h);
 }
 
+int lbm_set_index_width(lbm_handle *h, int32_t bits) {
+  if (!h) return fail(h, LBM_EINVAL, "lbm_set_index_width: null handle");
+  if (bits == 64) {
+    h->w64 = true;
+  } else if (bits == 32) {
+    if ((unsigned long long)h->plane * (unsigned long long)h->esize > 0xffffffffull)
+      return fail(h, LBM_EINVAL, "lbm_set_index_width: a population plane of this domain exceeds 4 GiB; 32-bit offsets cannot address it");
+    h->w64 = false;
+  } else {
+    return fail(h, LBM_EINVAL, "lbm_set_index_width: bits must be 32 or 64");
+  }
+  return LBM_OK;
+}
+
 int lbm_info(const lbm_handle *h, int64_t *ncell, int32_t *q, int32_t *bytes_per_pop) {
   if (!h) return fail(nullptr, LBM_EINVAL, "lbm_info: null handle");
   if (ncell) *ncell = h->ncell;
@@ -1064,3 +720,7 @@ int lbm_copy_bench(int64_t nbytes, int32_t reps, double *ms) {
 }
 
 }  // extern "C"
+
+// batched uneven domains (include/lbm_batch.h)
+#include "../../include/lbm_batch.h"
+#include "lbm_batch_impl.h"

@@ -1,0 +1,111 @@
+// Step kernels (single domain and batched) and their launchers, included
+// by the per-storage translation units lbm_step_<storage>.hip.
+#pragma once
+#include "lbm_launch.h"
+
+namespace lbm {
+
+// Occupancy target of the step kernel (waves per SIMD).  D3Q19 with FP32
+// storage (FP64 arithmetic) compiles to 129-131 VGPRs unconstrained, i.e. 3
+// waves; held to 128 it keeps 4 waves without spilling (32-bit offsets) and
+// runs 3-4 % faster.  FP64 storage spills under that limit and is left free.
+template <int Q, typename ST>
+struct StepWaves {
+  static constexpr int value = 1;
+};
+template <>
+struct StepWaves<19, float> {
+  static constexpr int value = 4;
+};
+
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<Q, ST>::value))) k_step(const KArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  double red[NSUM];
+#pragma unroll
+  for (int k = 0; k < NSUM; ++k) red[k] = 0;
+  cell_update<Q, ST, FLAGS, MODE, NT, W64>(a, x, y, z, red);
+  if (MODE >= 2) {
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
+    const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    block_reduce_store<256>(red, a.partials, b, nb);
+  }
+}
+
+template <int Q, typename ST, bool FLAGS, bool W64>
+static void launch_step_kernel(const KArgs &a, dim3 g, dim3 b, hipStream_t s, int mode, bool nt) {
+  if (mode == 0) {
+    if (nt) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, true, W64>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, false, W64>), g, b, 0, s, a);
+  } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 1, false, W64>), g, b, 0, s, a);
+  else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 2, false, W64>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 3, false, W64>), g, b, 0, s, a);
+}
+template <int Q, typename ST>
+void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64) {
+  if (w64) {
+    if (flags) launch_step_kernel<Q, ST, true, true>(a, g, b, s, mode, nt);
+    else launch_step_kernel<Q, ST, false, true>(a, g, b, s, mode, nt);
+  } else {
+    if (flags) launch_step_kernel<Q, ST, true, false>(a, g, b, s, mode, nt);
+    else launch_step_kernel<Q, ST, false, false>(a, g, b, s, mode, nt);
+  }
+}
+
+template <int Q, typename ST, int MODE, bool NT>
+__global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
+  using C = typename Store<ST>::C;
+  using V4 = typename Vec4<C>::T;
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const int4 tl = a.tiles[t];
+    if (!a.active[tl.x]) continue;  // uniform per block
+    const DomainDev &d = a.doms[tl.x];
+    BatchView v;
+    v.src = static_cast<const ST *>(a.src) + d.pop_off;
+    v.dst = static_cast<ST *>(a.dst) + d.pop_off;
+    v.flags = a.flags + d.cell_off;
+    v.plane = d.plane;
+    v.nx = d.nx;
+    v.ny = d.ny;
+    v.nz = d.nz;
+    v.px = d.px;
+    v.per[0] = d.per[0];
+    v.per[1] = d.per[1];
+    v.per[2] = d.per[2];
+    v.face = d.face;
+    v.omega = d.omega;
+    v.F[0] = v.F[1] = v.F[2] = 0.0;
+    v.snap_in = a.snap_in ? static_cast<V4 *>(a.snap_in) + d.cell_off : nullptr;
+    v.snap_out = a.snap_out ? static_cast<V4 *>(a.snap_out) + d.cell_off : nullptr;
+    const int x = tl.y + (int)threadIdx.x % a.tx;
+    const int y = tl.z + (int)threadIdx.x / a.tx;
+    const int z = tl.w & 0x3fffffff;
+    double red[NSUM];
+#pragma unroll
+    for (int k = 0; k < NSUM; ++k) red[k] = 0;
+    // tiles with no solid cell within one cell of them skip the flag loads
+    if (tl.w >> 30) cell_update<Q, ST, true, MODE, NT, false>(v, x, y, z, red);
+    else cell_update<Q, ST, false, MODE, NT, false>(v, x, y, z, red);
+    if (MODE >= 2) {
+      block_reduce_store<256>(red, a.partials, t, a.ntiles);
+      __syncthreads();  // the LDS of the reduction is reused by the next tile
+    }
+  }
+}
+
+template <int Q, typename ST>
+void BatchStepL<Q, ST>::run(const BatchArgs &a, int grid, hipStream_t s, int mode) {
+  if (mode == 0) hipLaunchKernelGGL((k_batch_step<Q, ST, 0, true>), dim3(grid), dim3(256), 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL((k_batch_step<Q, ST, 1, false>), dim3(grid), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_batch_step<Q, ST, 2, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
+}  // namespace lbm
+
+#define LBM_INSTANTIATE_STEPS(ST)              \
+  template struct lbm::StepL<9, ST>;           \
+  template struct lbm::StepL<19, ST>;          \
+  template struct lbm::BatchStepL<9, ST>;      \
+  template struct lbm::BatchStepL<19, ST>;

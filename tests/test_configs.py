@@ -1,0 +1,118 @@
+"""The benchmark configurations as tests.
+
+Config 1: 2D Poiseuille channel, D2Q9 BGK FP64, 256x64, tau 0.8, body force
+g_x = 1e-6, periodic x, halfway bounce-back walls at y rows 0 and 63, rest
+start, 20000 steps; checked against the transient start-up Fourier series
+(at t = 20000 the slowest mode still holds ~0.8 % of the centre-line velocity,
+so the steady parabola alone would not pin it).
+
+Config 2: 256^3 periodic D3Q19, tau 0.6, Taylor-Green u0 = 0.05 + Gaussian
+noise 1e-3 (splitmix64/Box-Muller, seed 1234), 1000 steps; FP32 and shifted
+FP16/BF16 storage against the FP64 path, whose agreement with the CPU oracle
+is pinned at 1e-11 on smaller boxes (tests/test_gpu_parity.py).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+NX, NY, TAU, G = 256, 64, 0.8, 1e-6
+NU = (2 * TAU - 1) / 6
+WALLS = {"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0)}
+
+
+def startup_series(t, H=NY, nmax=2001):
+    """u(y,t) = g/(2 nu) y (H - y) - sum_{n odd} 4 g H^2/(nu pi^3 n^3) sin(n pi y/H) exp(-nu n^2 pi^2 t/H^2),
+    walls halfway between nodes at y = 0 and y = H, node centres y = j + 1/2."""
+    y = np.arange(H) + 0.5
+    s = G / (2 * NU) * y * (H - y)
+    for n in range(1, nmax, 2):
+        s -= 4 * G * H * H / (NU * math.pi ** 3 * n ** 3) * np.sin(n * math.pi * y / H) * math.exp(
+            -NU * n * n * math.pi ** 2 * t / H ** 2)
+    return s
+
+
+def test_config1_oracle_against_startup_series():
+    p = O.Problem((NX, NY), faces=WALLS, omega=1 / TAU, force=(G, 0, 0))
+    N = NX * NY
+    p.init(np.ones(N), np.zeros((N, 2)))
+    p.step(20000)
+    rho, u = p.macros()
+    ux = u.reshape(NY, NX, 2)[:, :, 0]
+    assert np.abs(ux - ux[:, :1]).max() == 0.0  # exactly uniform along the periodic axis
+    a = startup_series(20000)
+    err = np.linalg.norm(ux[:, 0] - a) / np.linalg.norm(a)
+    y = np.arange(NY) + 0.5
+    steady = G / (2 * NU) * y * (NY - y)
+    err_steady = np.linalg.norm(ux[:, 0] - steady) / np.linalg.norm(steady)
+    assert err < 5e-4, err
+    assert err_steady > 5e-3  # the transient is resolved, not just the steady state
+    assert abs(rho.sum() - N) / N < 1e-11
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 5e-4)])
+def test_config1_gpu(storage, tol):
+    """Config 1 is an FP64 case.  FP32 storage is run for the record: the Guo
+    increments (~3e-7 per step on populations ~1e-2) sit a few hundred ulps
+    above FP32 resolution, so 20000 steps accumulate ~2e-4 relative error."""
+    from lbm_amd import capi
+
+    N = NX * NY
+    print(f"config1 gpu {storage}", flush=True)
+    lat = capi.Lattice((NX, NY), storage=storage, omega=1 / TAU, force=(G, 0, 0), faces=WALLS)
+    lat.init(np.ones(N), np.zeros((N, 2)))
+    lat.step(20000)
+    rho, u = lat.macros()
+    lat.close()
+    p = O.Problem((NX, NY), faces=WALLS, omega=1 / TAU, force=(G, 0, 0))
+    p.init(np.ones(N), np.zeros((N, 2)))
+    p.step(20000)
+    r_rho, r_u = p.macros()
+    eu = np.linalg.norm(u - r_u) / np.linalg.norm(r_u)
+    er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
+    a = startup_series(20000)
+    es = np.linalg.norm(u.reshape(NY, NX, 2)[:, 0, 0] - a) / np.linalg.norm(a)
+    print(f"config1 {storage}: vs oracle u={eu:.3e} rho={er:.3e}; vs start-up series {es:.3e}")
+    assert eu < tol and er < tol
+    assert es < 5e-4
+
+
+@pytest.mark.gpu
+def test_config2_full_size_storage_forms():
+    from lbm_amd import capi
+
+    n = 256
+    shape = (n, n, n)
+    rho0, u0 = O.taylor_green_noise(shape, 0.05, 1e-3, seed=1234)
+    res = {}
+    for storage in ("f64", "f32", "f16", "bf16"):
+        print(f"config2 {storage}", flush=True)
+        lat = capi.Lattice(shape, storage=storage, omega=1 / 0.6)
+        lat.init(rho0, u0)
+        s0 = lat.reduce()
+        lat.step(1000)
+        s1 = lat.reduce()
+        rho, u = lat.macros()
+        lat.close()
+        res[storage] = (rho.astype(np.float64), u.astype(np.float64), s0, s1)
+    r_rho, r_u, s0, s1 = res["f64"]
+    # mass is conserved by every form up to its storage rounding
+    assert abs(s1.mass - s0.mass) / s0.mass < 1e-12
+    ke0 = (u0 ** 2).sum()
+    ke = (r_u ** 2).sum()
+    nu = (2 * 0.6 - 1) / 6
+    k2 = 3 * (2 * np.pi / n) ** 2
+    decay = ke / ke0 / math.exp(-2 * nu * k2 * 1000)
+    print(f"config2 FP64 kinetic-energy decay / linear theory = {decay:.5f}")
+    assert abs(decay - 1) < 0.02
+    bounds = {"f32": 1e-5, "f16": 2e-3, "bf16": 3e-2}
+    for st in ("f32", "f16", "bf16"):
+        rho, u, _, s1b = res[st]
+        eu = np.linalg.norm(u - r_u) / np.linalg.norm(r_u)
+        er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
+        dm = abs(s1b.mass - s0.mass) / s0.mass
+        print(f"config2 {st}: rel-L2 vs FP64 u={eu:.3e} rho={er:.3e}, mass drift {dm:.2e}")
+        assert eu < bounds[st] and er < bounds[st]
