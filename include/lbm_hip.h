@@ -47,6 +47,12 @@ extern "C" {
 #define LBM_FACE_PRESSURE_ABB 2 /* 'pressure' anti-bounce-back (src/lbm/bnd/bnd_pressure.h:55-106) */
 #define LBM_FACE_VELOCITY_ZH 3 /* Zou-He velocity inlet (no reference counterpart) */
 #define LBM_FACE_PRESSURE_ZH 4 /* Zou-He pressure outlet (no reference counterpart) */
+#define LBM_FACE_PRESSURE_NEEM 5 /* non-equilibrium extrapolation pressure face (Guo, Zheng & Shi 2002):
+                                    every population of a face cell from its inward neighbour (no
+                                    reference counterpart; the faces next to it must be walls or
+                                    periodic) */
+#define LBM_FACE_VELOCITY_NEEM 6 /* non-equilibrium extrapolation velocity face, density from the inward
+                                    neighbour (no reference counterpart; same neighbour rule) */
 
 /* cell flags (optional byte per cell) */
 #define LBM_CELL_FLUID 0

@@ -68,6 +68,7 @@ struct lbm_batch {
   void *buf[2] = {nullptr, nullptr};
   void *snap[2] = {nullptr, nullptr};
   uint8_t *flags = nullptr;
+  uint32_t *links = nullptr;  // solid-link masks, same layout as flags
   DomainDev *doms_d = nullptr;
   int4 *tiles_d = nullptr;
   long long ntiles = 0;
@@ -151,7 +152,7 @@ static BatchArgs batch_args(lbm_batch *b, int parity, int mode) {
   memset(&a, 0, sizeof(a));
   a.src = b->buf[parity];
   a.dst = b->buf[parity ^ 1];
-  a.flags = b->flags;
+  a.links = b->links;
   a.doms = b->doms_d;
   a.tiles = b->tiles_d;
   a.ntiles = (int)b->ntiles;
@@ -224,7 +225,7 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
     }
     for (int i = 0; i < dd.nfaces; ++i) {
       const lbm_bc_desc &f = dd.faces[i];
-      if (f.face < 0 || f.face >= 2 * b->dim || f.kind < 0 || f.kind > LBM_FACE_PRESSURE_ZH) {
+      if (f.face < 0 || f.face >= 2 * b->dim || f.kind < 0 || f.kind > LBM_FACE_VELOCITY_NEEM) {
         bfail(nullptr, LBM_EINVAL, "lbm_batch_create: bad face in domain " + std::to_string(d));
         return bail(LBM_EINVAL);
       }
@@ -240,6 +241,10 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
         return bail(LBM_EINVAL);
       }
       D.per[ax] = p0 ? 1 : 0;
+    }
+    if (!neem_neighbours_ok(D.face, b->dim)) {
+      bfail(nullptr, LBM_EINVAL, "lbm_batch_create: the faces next to a NEEM face must be walls or periodic");
+      return bail(LBM_EINVAL);
     }
     const long long nc = D.plane;
     flags.resize(cells + nc, LBM_CELL_SOLID);
@@ -270,6 +275,13 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
   if (hipMalloc(&b->snap[0], (size_t)cells * cs) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "snapshot"));
   if (hipMalloc(&b->flags, flags.size()) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "flags"));
   if (hipMemcpy(b->flags, flags.data(), flags.size(), hipMemcpyHostToDevice) != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "flags"));
+  if (hipMalloc(&b->links, flags.size() * sizeof(uint32_t)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "link masks"));
+  for (int d = 0; d < ndom; ++d) {
+    const DomainDev &D = b->doms[d];
+    if (b->Q == 9) launch_links<9>(b->flags + D.cell_off, b->links + D.cell_off, D.nx, D.ny, D.nz, D.px, D.per, b->stream);
+    else launch_links<19>(b->flags + D.cell_off, b->links + D.cell_off, D.nx, D.ny, D.nz, D.px, D.per, b->stream);
+  }
+  if (hipGetLastError() != hipSuccess) return bail(bfail(nullptr, LBM_EHIP, "link masks"));
   if (hipMalloc(&b->doms_d, ndom * sizeof(DomainDev)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "domain table"));
   if (hipMalloc(&b->active_d, ndom * sizeof(int)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "active"));
   if (hipMalloc(&b->res_d, ndom * sizeof(double)) != hipSuccess) return bail(bfail(nullptr, LBM_ENOMEM, "res"));
@@ -514,7 +526,7 @@ int lbm_batch_destroy(lbm_batch *b) {
     if (b->buf[k]) (void)hipFree(b->buf[k]);
     if (b->snap[k]) (void)hipFree(b->snap[k]);
   }
-  void *ptrs[] = {b->flags, b->doms_d, b->tiles_d, b->active_d, b->res_d, b->mass_d, b->partials, b->conv_d, b->cycle_d};
+  void *ptrs[] = {b->flags, b->links, b->doms_d, b->tiles_d, b->active_d, b->res_d, b->mass_d, b->partials, b->conv_d, b->cycle_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);

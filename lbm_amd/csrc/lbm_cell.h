@@ -24,10 +24,17 @@ struct FaceDev {
   double rho;
 };
 
+// Solid-link mask of a cell (padded layout): bit j (j < Q-1) is set when the
+// upstream cell x - c_j (periodic wrap applied) is solid; bit 31 when the
+// cell itself is solid.  One coalesced 4-byte load replaces the Q-1 flag
+// loads of the neighbours in the step kernels.
+constexpr uint32_t LINK_SELF_SOLID = 0x80000000u;
+
 struct KArgs {
   const void *src;
   void *dst;
-  const uint8_t *flags;  // padded layout, may be null
+  const uint8_t *flags;    // padded layout, may be null
+  const uint32_t *links;   // solid-link masks (padded layout), null without solids
   long long plane;       // elements between population planes
   int nx, ny, nz, px;
   int per[3];
@@ -232,12 +239,84 @@ __device__ void cell_velocity(const ST *src, long long plane, long long cell, do
   u[0] = m.ux; u[1] = m.uy; u[2] = m.uz;
 }
 
+// Non-equilibrium extrapolation faces (Guo, Zheng & Shi 2002): all
+// populations of a face cell are rebuilt from its inward neighbour n1,
+//   f_i = f_i(n1) + feq_i(rho_b, u_b) - feq_i(rho1, u1),
+// with (rho_b, u_b) = (rho_face, u1) on a pressure face and (rho1, u_face)
+// on a velocity face; in shifted form h_i = h_i(n1) + w_i (rho_b P_i(u_b) -
+// rho1 P_i(u1)), P_i(u) = 1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2.  The
+// populations of n1 are its post-streaming set: gathered from its upstream
+// cells, bounced back where one is missing (walls with the Ladd term,
+// solids plain; faces next to a NEEM face are walls or periodic, checked on
+// the host).  With a solid n1 the cell keeps its bounce-back values.
+template <int Q, typename ST, bool FLAGS, typename G>
+__device__ void neem_face(const G &a, int x, int y, int z, int f, typename Store<ST>::R (&h)[Q]) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  using R = typename Store<ST>::R;
+  const ST *src = static_cast<const ST *>(a.src);
+  const long long plane = a.plane;
+  const int ax = f >> 1, in = (f & 1) ? -1 : 1;
+  const int x1 = x + (ax == 0 ? in : 0), y1 = y + (ax == 1 ? in : 0), z1 = z + (ax == 2 ? in : 0);
+  const long long n1 = (long long)x1 + (long long)a.px * ((long long)y1 + (long long)a.ny * z1);
+  const uint32_t lm1 = FLAGS ? a.links[n1] : 0u;
+  if (lm1 & LINK_SELF_SOLID) return;
+  // unrolled so that the flag and population loads of all directions are
+  // in flight together (face cells are one lane of their wave on x faces)
+  const int ext[3] = {a.nx, a.ny, a.nz};
+  int face_of[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int cj[3] = {L::cx[j], L::cy[j], L::cz[j]};
+    int s[3] = {x1 - cj[0], y1 - cj[1], z1 - cj[2]};
+    int face = -1;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) {
+      if (cj[d] == 0) continue;
+      if (s[d] >= 0 && s[d] < ext[d]) continue;
+      if (a.per[d]) { s[d] = s[d] < 0 ? s[d] + ext[d] : s[d] - ext[d]; continue; }
+      ok = false;
+      const int g = 2 * d + (cj[d] > 0 ? 0 : 1);
+      if (face < 0 || a.face[g].prio > a.face[face].prio) face = g;
+    }
+    const long long t = (long long)s[0] + (long long)a.px * ((long long)s[1] + (long long)a.ny * s[2]);
+    if (ok && FLAGS && j != Q - 1) ok = !((lm1 >> j) & 1u);
+    const long long addr = ok ? (long long)j * plane + t : (long long)L::opp[j] * plane + n1;
+    h[j] = Store<ST>::ld(src + addr);
+    face_of[j] = ok ? -2 : face;  // -2: streamed, -1: solid (plain bounce-back)
+  }
+#pragma unroll
+  for (int j = 0; j < Q - 1; ++j) {
+    const int face = face_of[j];
+    if (face >= 0 && a.face[face].kind == LBM_FACE_WALL)
+      h[j] = R(C(h[j]) + C(6) * C(L::w[j]) *
+                             (C(L::cx[j]) * C(a.face[face].u[0]) + C(L::cy[j]) * C(a.face[face].u[1]) +
+                              C(L::cz[j]) * C(a.face[face].u[2])));
+  }
+  const Moments<Q, C> m = moments<Q, C>(h, C(a.F[0]), C(a.F[1]), C(a.F[2]));
+  const bool vel = a.face[f].kind == LBM_FACE_VELOCITY_NEEM;
+  const C rb = vel ? m.rho : C(a.face[f].rho);
+  const C bx = vel ? C(a.face[f].u[0]) : m.ux, by = vel ? C(a.face[f].u[1]) : m.uy,
+          bz = vel ? C(a.face[f].u[2]) : m.uz;
+  const C usq1 = m.ux * m.ux + m.uy * m.uy + m.uz * m.uz;
+  const C usqb = bx * bx + by * by + bz * bz;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const C cu1 = C(L::cx[j]) * m.ux + C(L::cy[j]) * m.uy + C(L::cz[j]) * m.uz;
+    const C cub = C(L::cx[j]) * bx + C(L::cy[j]) * by + C(L::cz[j]) * bz;
+    const C pb = C(1) + C(3) * cub + C(4.5) * cub * cub - C(1.5) * usqb;
+    const C p1 = C(1) + C(3) * cu1 + C(4.5) * cu1 * cu1 - C(1.5) * usq1;
+    h[j] = R(C(h[j]) + C(L::w[j]) * (rb * pb - m.rho * p1));
+  }
+}
+
 // --------------------------------------------------------------------------
 // The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
 // 2: reduce against the snapshot written by the previous step; 3: both
 // (reduce against snap_in, write snap_out).
 // Per-cell update shared by the single-domain and batched kernels.  G is a
-// view of one structured domain with members src, dst, flags, plane, nx, ny,
+// view of one structured domain with members src, dst, links, plane, nx, ny,
 // nz, px, per[3], face[6], omega, F[3], snap_in, snap_out (the kernel
 // arguments for one domain, or a domain of a batch).  red must be zeroed by
 // the caller; MODE >= 2 accumulates this cell's sums into it.
@@ -251,7 +330,8 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
   const long long plane = a.plane;
   const I cell = I(x) + I(a.px) * (I(y) + I(a.ny) * I(z));
   bool active = x < a.nx && y < a.ny;
-  if (FLAGS && active) active = a.flags[cell] == LBM_CELL_FLUID;
+  const uint32_t lm = (FLAGS && active) ? a.links[cell] : 0u;
+  if (lm & LINK_SELF_SOLID) active = false;
 
   if (active) {
     // neighbour coordinates per axis with periodic wrap; v* = exists
@@ -277,19 +357,21 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
       bool valid = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true)) &&
                    (cz > 0 ? vzm : (cz < 0 ? vzp : true));
       const I s = I(sx) + I(a.px) * (I(sy) + I(a.ny) * I(sz));
-      // A population streams iff its upstream cell is fluid.  This is the
-      // symmetric rule; the reference composes diagonal neighbours from main
-      // ones (src/cartesiangrid.h:466-486), which next to staircase obstacles
-      // makes the neighbour relation one-sided and leaves some populations
-      // unwritten for a step.
-      if (FLAGS && valid && j != Q - 1) valid = a.flags[s] == LBM_CELL_FLUID;
-      // a missing upstream cell loads the cell's own entry of plane j (in
-      // bounds, same cache lines as the neighbours' loads) and is replaced
-      // below, so every load of the loop keeps the uniform base of plane j
+      // a missing upstream cell (across a face) loads the cell's own entry
+      // of plane j (in bounds, same cache lines as the neighbours' loads)
+      // and is replaced below, so every load of the loop keeps the uniform
+      // base of plane j
       const ST *p = pop_ptr<ST, W64>(src, plane, j, valid ? s : cell);
       h[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
       if (!valid) missing |= 1u << j;
     }
+    // A population streams iff its upstream cell is fluid.  This is the
+    // symmetric rule; the reference composes diagonal neighbours from main
+    // ones (src/cartesiangrid.h:466-486), which next to staircase obstacles
+    // makes the neighbour relation one-sided and leaves some populations
+    // unwritten for a step.  Populations pulled from a solid cell (link
+    // bits) are replaced below like the ones across a face.
+    if (FLAGS) missing |= lm & ((1u << (Q - 1)) - 1u);
 
     const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
     const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
@@ -341,17 +423,22 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
         }
         // Zou-He faces: overwritten below
       }
-      // Zou-He on the first Zou-He face the cell lies on
+      // Zou-He or NEEM on the first such face the cell lies on
 #pragma unroll 1
       for (int f = 0; f < 2 * L::D; ++f) {
         const int k = a.face[f].kind;
-        if (k != LBM_FACE_VELOCITY_ZH && k != LBM_FACE_PRESSURE_ZH) continue;
+        if (k != LBM_FACE_VELOCITY_ZH && k != LBM_FACE_PRESSURE_ZH && k != LBM_FACE_PRESSURE_NEEM &&
+            k != LBM_FACE_VELOCITY_NEEM)
+          continue;
         const int ax = f >> 1;
         const int coord = ax == 0 ? x : (ax == 1 ? y : z);
         const int n = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
         if (coord != ((f & 1) ? n - 1 : 0)) continue;
-        zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
-                     R(a.face[f].u[2]), R(a.face[f].rho));
+        if (k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM)
+          neem_face<Q, ST, FLAGS>(a, x, y, z, f, h);
+        else
+          zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
+                       R(a.face[f].u[2]), R(a.face[f].rho));
         break;
       }
     }

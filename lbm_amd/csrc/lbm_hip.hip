@@ -53,6 +53,31 @@ __global__ void __launch_bounds__(1024) k_finalize(const double *partials, long 
   if (threadIdx.x == 0) out[k] = s[0];
 }
 
+// Solid-link masks (see LINK_SELF_SOLID) of a padded box; the periodic
+// axes wrap, links across a non-periodic face stay clear (faces are handled
+// from coordinates).  One thread per padded cell.
+template <int Q>
+__global__ void __launch_bounds__(256) k_links(const uint8_t *flags, uint32_t *links, int nx, int ny, int nz, int px,
+                                               int p0, int p1, int p2) {
+  using L = Lat<Q>;
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= px) return;
+  const long long cell = x + (long long)px * (y + (long long)ny * z);
+  uint32_t m = 0;
+  if (x < nx) {
+    if (flags[cell] != LBM_CELL_FLUID) m = LINK_SELF_SOLID;
+#pragma unroll
+    for (int j = 0; j < Q - 1; ++j) {
+      int sx = x - L::cx[j], sy = y - L::cy[j], sz = z - L::cz[j];
+      if (sx < 0 || sx >= nx) { if (!p0) continue; sx = sx < 0 ? sx + nx : sx - nx; }
+      if (sy < 0 || sy >= ny) { if (!p1) continue; sy = sy < 0 ? sy + ny : sy - ny; }
+      if (sz < 0 || sz >= nz) { if (!p2) continue; sz = sz < 0 ? sz + nz : sz - nz; }
+      if (flags[sx + (long long)px * (sy + (long long)ny * sz)] != LBM_CELL_FLUID) m |= 1u << j;
+    }
+  }
+  links[cell] = m;
+}
+
 // Initial condition: h = feq(rho, u) - w, then the first collision, i.e. the
 // state after the collision of the reference's first time step.
 template <int Q, typename ST>
@@ -195,6 +220,7 @@ struct lbm_handle {
   void *buf[2] = {nullptr, nullptr};
   int cur = 0;
   uint8_t *flags = nullptr;
+  uint32_t *links = nullptr;  // solid-link masks (padded layout)
   bool has_flags = false;
   FaceDev face[6];
   int per[3] = {1, 1, 1};
@@ -232,6 +258,7 @@ static KArgs make_args(lbm_handle *h) {
   a.src = h->buf[h->cur];
   a.dst = h->buf[h->cur ^ 1];
   a.flags = h->has_flags ? h->flags : nullptr;
+  a.links = h->has_flags ? h->links : nullptr;
   a.plane = h->plane;
   a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.px = h->px;
   for (int i = 0; i < 3; ++i) a.per[i] = h->per[i];
@@ -321,6 +348,23 @@ static int ensure_partials(lbm_handle *h) {
   return LBM_OK;
 }
 
+template <int Q>
+static void launch_links(const uint8_t *flags, uint32_t *links, int nx, int ny, int nz, int px, const int *per,
+                         hipStream_t st) {
+  hipLaunchKernelGGL((k_links<Q>), dim3((px + 255) / 256, ny, nz), dim3(256), 0, st, flags, links, nx, ny, nz, px,
+                     per[0], per[1], per[2]);
+}
+
+// (Re)build the solid-link masks; they depend on the periodic axes, so
+// lbm_set_bc calls this again.
+static int build_links(lbm_handle *h) {
+  if (!h->has_flags) return LBM_OK;
+  if (h->Q == 9) launch_links<9>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
+  else launch_links<19>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
+  HIPCHK(h, hipGetLastError());
+  return LBM_OK;
+}
+
 extern "C" {
 
 int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out) {
@@ -386,12 +430,31 @@ int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out)
       fail(nullptr, LBM_EHIP, "lbm_create: flag upload failed");
       return bail(LBM_EHIP);
     }
+    const long long ncp = (long long)h->px * h->ny * h->nz;
+    if (hipMalloc(&h->links, ncp * sizeof(uint32_t)) != hipSuccess) {
+      fail(nullptr, LBM_ENOMEM, "lbm_create: hipMalloc of link masks failed");
+      return bail(LBM_ENOMEM);
+    }
     h->has_flags = true;
+    rc = build_links(h);
+    if (rc) return bail(rc);
   }
   rc = ensure_partials(h);
   if (rc) return bail(rc);
   *out = h;
   return LBM_OK;
+}
+
+// A NEEM face reads the post-streaming populations of the inward
+// neighbour with bounce-back only, so the faces of the other axes must be
+// walls or periodic.
+static bool neem_neighbours_ok(const FaceDev *face, int dim) {
+  for (int f = 0; f < 2 * dim; ++f) {
+    if (face[f].kind != LBM_FACE_PRESSURE_NEEM && face[f].kind != LBM_FACE_VELOCITY_NEEM) continue;
+    for (int g = 0; g < 2 * dim; ++g)
+      if ((g >> 1) != (f >> 1) && face[g].kind != LBM_FACE_PERIODIC && face[g].kind != LBM_FACE_WALL) return false;
+  }
+  return true;
 }
 
 int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
@@ -404,7 +467,7 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const lbm_bc_desc &d = bcs[i];
     if (d.face < 0 || d.face >= 2 * h->dim) return fail(h, LBM_EINVAL, "lbm_set_bc: face index out of range");
     if (seen[d.face]) return fail(h, LBM_EINVAL, "lbm_set_bc: face described twice");
-    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_PRESSURE_ZH) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
+    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_VELOCITY_NEEM) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
     seen[d.face] = true;
     face[d.face].kind = d.kind;
     face[d.face].prio = d.priority;
@@ -418,13 +481,15 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const int na = ax == 0 ? h->nx : (ax == 1 ? h->ny : h->nz);
     for (int s = 0; s < 2; ++s) {
       const int k = face[2 * ax + s].kind;
-      if (k == LBM_FACE_PRESSURE_ABB && na < 3)
-        return fail(h, LBM_EINVAL, "lbm_set_bc: pressure face needs at least 3 cells along its normal");
+      if ((k == LBM_FACE_PRESSURE_ABB || k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM) && na < 3)
+        return fail(h, LBM_EINVAL, "lbm_set_bc: anti-bounce-back and NEEM faces need at least 3 cells along their normal");
     }
   }
+  if (!neem_neighbours_ok(face, h->dim))
+    return fail(h, LBM_EINVAL, "lbm_set_bc: the faces next to a NEEM face must be walls or periodic");
   memcpy(h->face, face, sizeof(face));
   memcpy(h->per, per, sizeof(per));
-  return LBM_OK;
+  return build_links(h);
 }
 
 int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
@@ -679,6 +744,7 @@ int lbm_destroy(lbm_handle *h) {
   for (int b = 0; b < 2; ++b)
     if (h->buf[b]) hipFree(h->buf[b]);
   if (h->flags) hipFree(h->flags);
+  if (h->links) hipFree(h->links);
   for (int b = 0; b < 2; ++b)
     if (h->snap[b]) hipFree(h->snap[b]);
   if (h->partials) hipFree(h->partials);

@@ -25,7 +25,7 @@ struct DomainDev {
 struct BatchView {
   const void *src;
   void *dst;
-  const uint8_t *flags;
+  const uint32_t *links;
   long long plane;
   int nx, ny, nz, px;
   int per[3];
@@ -39,7 +39,7 @@ struct BatchView {
 struct BatchArgs {
   const void *src;  // buffer base pointers of this step
   void *dst;
-  const uint8_t *flags;
+  const uint32_t *links;  // solid-link masks of all domains (cell_off layout)
   const DomainDev *doms;
   const int4 *tiles;
   int ntiles;

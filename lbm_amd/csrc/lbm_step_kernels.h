@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
     BatchView v;
     v.src = static_cast<const ST *>(a.src) + d.pop_off;
     v.dst = static_cast<ST *>(a.dst) + d.pop_off;
-    v.flags = a.flags + d.cell_off;
+    v.links = a.links + d.cell_off;
     v.plane = d.plane;
     v.nx = d.nx;
     v.ny = d.ny;

@@ -40,6 +40,8 @@ void LBMBndManager::setupBndryCnds(const json& bndConfig, CartesianGrid& grid) {
         const auto model = opt_value<std::string>(props, "model", "antibounceback");
         if (model == "zouhe") {
           m_bndrys.emplace_back(std::make_unique<LBMBnd_ZouHe>(&srf, props, dim, false));
+        } else if (model == "neem") {
+          m_bndrys.emplace_back(std::make_unique<LBMBnd_NEEM>(&srf, props, dim, false));
         } else {
           auto p = std::make_unique<LBMBnd_Pressure>(&srf, props);
           p->setRhoVar(dim);
@@ -53,6 +55,8 @@ void LBMBndManager::setupBndryCnds(const json& bndConfig, CartesianGrid& grid) {
           m_bndrys.emplace_back(std::make_unique<LBMBnd_DirichletBB>(&srf, props, dim));
         } else if (model == "zouhe") {
           m_bndrys.emplace_back(std::make_unique<LBMBnd_ZouHe>(&srf, props, dim, true));
+        } else if (model == "neem") {
+          m_bndrys.emplace_back(std::make_unique<LBMBnd_NEEM>(&srf, props, dim, true));
         } else {
           TERMM(-1, "Dirichlet boundary model '" + model + "' is not part of this build");
         }

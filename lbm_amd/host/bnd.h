@@ -195,6 +195,45 @@ class LBMBnd_ZouHe : public LBMBndInterface {
   GDouble m_p = 1.0;
 };
 
+// Non-equilibrium extrapolation faces: 'pressure' with model 'neem'
+// (prescribed density) and 'dirichlet' with model 'neem' (prescribed
+// velocity).  New in this build.  Every population of a face cell is rebuilt
+// from its inward neighbour.
+class LBMBnd_NEEM : public LBMBndInterface {
+ public:
+  LBMBnd_NEEM(const Surface* s, const json& props, GInt dim, bool velocity) : m_s(s), m_dim(dim), m_vel(velocity) {
+    if (velocity) {
+      auto v = required_value<std::vector<GDouble>>(props, "value");
+      for (GInt d = 0; d < dim && d < (GInt)v.size(); ++d) m_value[d] = v[d];
+    } else {
+      m_p = required_value<GDouble>(props, "pressure");
+    }
+  }
+  void initCnd(const VarAccess& vars) override {
+    for (GInt c : m_s->getCellList()) {
+      if (m_vel)
+        for (GInt d = 0; d < m_dim; ++d) vars(c, d) = m_value[d];
+      else
+        vars(c, m_dim) = m_p;
+    }
+  }
+  [[nodiscard]] auto lower(GInt) const -> BndRule override {
+    BndRule r;
+    r.kind = m_vel ? LBM_FACE_VELOCITY_NEEM : LBM_FACE_PRESSURE_NEEM;
+    r.value = m_value;
+    r.rho = m_p;
+    return r;
+  }
+  [[nodiscard]] auto surface() const -> const Surface* override { return m_s; }
+
+ private:
+  const Surface* m_s;
+  GInt m_dim;
+  bool m_vel;
+  std::array<GDouble, 3> m_value{{0, 0, 0}};
+  GDouble m_p = 1.0;
+};
+
 // Device plan compiled from the boundary objects.
 struct DevicePlan {
   GInt nx = 1, ny = 1, nz = 1;

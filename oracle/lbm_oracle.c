@@ -152,6 +152,9 @@ int orc_macros(int Q, int64_t ncell, const double *fold, const double *force, do
  *          1.5*u(n1) - 0.5*u(n2), bnd_pressure.h:55-106
  *   3 ZHV  Zou-He velocity face (new): outward normal axis/sign, velocity u
  *   4 ZHP  Zou-He pressure face (new): outward normal axis/sign, density rho
+ *   5 NEEM non-equilibrium extrapolation pressure face (new): all
+ *          populations from the inward neighbour n1, density rho
+ *   6 NEEM non-equilibrium extrapolation velocity face (new), velocity u
  * dirmask bit i: apply to lattice direction i (the direction of the missing
  * neighbour), only consulted by BB and ABB. */
 typedef struct {
@@ -290,6 +293,35 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
       }
     } else if (r->kind == 3 || r->kind == 4) {
       zou_he_cell(&L, fold + c * Q, r->axis, r->sign, r->kind == 3, r->u, r->rho);
+    } else if (r->kind == 5 || r->kind == 6) {
+      /* NEEM faces (Guo, Zheng & Shi, Chin. Phys. 11 366 (2002)): every
+       * population of c from the inward neighbour n1 (complete after the
+       * bounce-back records),
+       *   f_i(c) = f_i(n1) + feq_i(rho_b, u_b) - feq_i(rho1, u1),
+       * rho1, u1 the moments of f(n1) (u1 with the Guo half-force shift);
+       * pressure face (5): rho_b = rho, u_b = u1; velocity face (6):
+       * rho_b = rho1, u_b = u. */
+      const double *f1 = fold + r->n1 * Q;
+      double v1[4];
+      macros_cell(&L, f1, has_force ? force : NULL, v1);
+      const double rho1 = v1[L.dim];
+      const double rhob = r->kind == 5 ? r->rho : rho1;
+      const double *ub = r->kind == 5 ? v1 : r->u;
+      double vsq1 = 0, vsqb = 0;
+      for (int d = 0; d < L.dim; ++d) {
+        vsq1 += v1[d] * v1[d];
+        vsqb += ub[d] * ub[d];
+      }
+      double fn[19];
+      for (int i = 0; i < Q; ++i) {
+        double cu1 = 0, cub = 0;
+        for (int d = 0; d < L.dim; ++d) {
+          cu1 += L.c[i][d] * v1[d];
+          cub += L.c[i][d] * ub[d];
+        }
+        fn[i] = f1[i] + default_eq(L.w[i], rhob, cub, vsqb) - default_eq(L.w[i], rho1, cu1, vsq1);
+      }
+      for (int i = 0; i < Q; ++i) fold[c * Q + i] = fn[i];
     } else {
       return -2;
     }

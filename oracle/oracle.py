@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblbm_oracle.so")
 REF_HILBERT = os.path.join(HERE, "_ref", "libref_hilbert.so")
 
-FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4}
+FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6}
 
 
 class OrcBC(ct.Structure):
@@ -201,18 +201,34 @@ class Problem:
                     r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
                     r.n2 = c2[0] + nx * (c2[1] + ny * c2[2])
                     recs.append(r)
-            # Zou-He: first Zou-He face the cell lies on
+                elif kind in (FACE["pressure_neem"], FACE["velocity_neem"]):
+                    # the face record (below) replaces every population; with
+                    # a solid inward neighbour the missing ones bounce back
+                    step = -1 if side == 1 else 1
+                    c1 = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+                    c1[ax] += step
+                    if self.solid[c1[0] + nx * (c1[1] + ny * c1[2])]:
+                        r.kind = 1
+                        recs.append(r)
+            # Zou-He / NEEM: first such face the cell lies on
             co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
             for f in range(2 * dim):
                 k = self.face_kind[f]
-                if k not in (FACE["velocity_zh"], FACE["pressure_zh"]):
+                if k not in (FACE["velocity_zh"], FACE["pressure_zh"], FACE["pressure_neem"], FACE["velocity_neem"]):
                     continue
                 ax, side = f // 2, f % 2
                 if co[ax] != (ext[ax] - 1 if side else 0):
                     continue
                 r = OrcBC()
                 r.cell = int(cidx)
-                r.kind = 3 if k == FACE["velocity_zh"] else 4
+                r.kind = {FACE["velocity_zh"]: 3, FACE["pressure_zh"]: 4, FACE["pressure_neem"]: 5,
+                          FACE["velocity_neem"]: 6}[k]
+                if k in (FACE["pressure_neem"], FACE["velocity_neem"]):
+                    c1 = list(co)
+                    c1[ax] += -1 if side else 1
+                    r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
+                    if self.solid[r.n1]:
+                        break  # bounce-back records above
                 r.axis = ax
                 r.sign = 1 if side else -1
                 r.u = (ct.c_double * 3)(*self.face_u[f])

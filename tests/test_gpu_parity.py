@@ -257,3 +257,44 @@ def test_index_width_rejects_bad_values():
     with pytest.raises(capi.LBMError):
         lat.set_index_width(16)
     lat.close()
+
+
+def cylinder_solid2(shape, c, r):
+    nx, ny = shape
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    return (((xx + 0.5 - c[0]) ** 2 + (yy + 0.5 - c[1]) ** 2) < r * r).reshape(-1)
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d2q9_neem_outlet_cylinder(storage, tol):
+    shape = (60, 22)
+    solid = cylinder_solid2(shape, (18, 11), 3.5)
+    faces = {"-y": ("wall", None, 1.0), "+y": ("wall", (0.01, 0.0), 1.0), "-x": ("velocity_neem", (0.04, 0.003), 1.0),
+             "+x": ("pressure_neem", None, 1.002)}
+    eu, er, _ = run_pair(shape, 300, storage, 1 / 0.56, faces=faces, solid=solid, seed=3)
+    print(f"d2q9 NEEM outlet {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+def test_d3q19_sphere_channel_neem_outlet(storage, tol):
+    shape = (40, 16, 14)
+    solid = sphere_solid(shape, (12, 8, 7), 3.5)
+    f = channel_faces3()
+    f["-x"] = ("velocity_neem", (0.04, 0.0, 0.0), 1.0)
+    f["+x"] = ("pressure_neem", None, 1.0)
+    f["+z"] = ("wall", (0.02, 0.0, 0.0), 1.0)  # moving wall next to the NEEM face
+    N = int(np.prod(shape))
+    u0 = np.zeros((N, 3))
+    u0[~solid, 0] = 0.04
+    eu, er, _ = run_pair(shape, 150, storage, 1 / 0.5384, faces=f, solid=solid, init=(np.ones(N), u0))
+    print(f"d3q19 sphere NEEM outlet {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
+
+
+def test_neem_next_to_open_face_rejected():
+    lat = capi.Lattice((12, 8), storage="f32", omega=1.0)
+    with pytest.raises(capi.LBMError):
+        lat.set_faces({"-x": ("velocity_zh", (0.02, 0.0), 1.0), "+x": ("pressure_neem", None, 1.0),
+                       "-y": ("pressure_abb", None, 1.0), "+y": ("wall", None, 1.0)})
+    lat.close()
