@@ -175,6 +175,65 @@ def run_sphere(args):
     print(json.dumps(line), flush=True)
 
 
+def run_warmstart(args):
+    """Config 5: learned warm start on 256x128 D2Q9 obstacle channels
+    (1-4 circles/rectangles, Re 20..200 on the channel height, u_in = 0.05).
+    Training fields come from the batched FP32 solver (relative residual
+    1e-5); the surrogate (bf16 MFMA U-Net, ~2M parameters) is trained on
+    them; on unseen channels the plain FP32 start (rho = 1, inlet velocity in
+    every fluid cell) and the warm start both run to the relative residual
+    1e-6, and their velocities are compared with FP64 solutions converged to
+    1e-10."""
+    import numpy as np
+
+    from lbm_amd import surrogate as S
+    from lbm_amd import warmstart as WS
+
+    def log(msg):
+        print(msg, flush=True)
+
+    t0 = time.perf_counter()
+    doms, x = WS.make_set(args.ntrain, seed=11)
+    steps, res, wall_gen, _, fields = WS.solve(doms, 1e-5, args.max_steps)
+    ok = steps > 0
+    log(f"# training set: {int(ok.sum())}/{len(doms)} channels converged to 1e-5 in {wall_gen:.1f} s "
+        f"(steps {int(steps[ok].min())}..{int(steps[ok].max())})")
+    keep = np.nonzero(ok)[0]
+    t, m = WS.targets([doms[i] for i in keep], [fields[i] for i in keep])
+    sur = S.Surrogate(batch=16, height=WS.NY, width=WS.NX, seed=7)
+    losses, wall_train = WS.train(sur, x[keep], t, m, args.epochs, 1e-3, 2e-5, log=log)
+    tdoms, tx = WS.make_set(args.ntest, seed=12)
+    cold_steps, _, cold_wall, cold_dev, cold_fields = WS.solve(tdoms, 1e-6, args.max_steps)
+    pred, wall_pred = WS.predict_fields(sur, tx, tdoms)
+    warm_steps, _, warm_wall, warm_dev, warm_fields = WS.solve(tdoms, 1e-6, args.max_steps, fields=pred)
+    ref_steps, _, _, _, ref_fields = WS.solve(tdoms, 1e-10, 4 * args.max_steps, storage="f64", fields=cold_fields)
+    both = (cold_steps > 0) & (warm_steps > 0) & (ref_steps > 0)
+    e_cold, e_warm, e_pred = [], [], []
+    for i in np.nonzero(both)[0]:
+        sd = tdoms[i]["solid"]
+        e_cold.append(WS.rel_l2(cold_fields[i][1], ref_fields[i][1], sd))
+        e_warm.append(WS.rel_l2(warm_fields[i][1], ref_fields[i][1], sd))
+        e_pred.append(WS.rel_l2(pred[i][1], ref_fields[i][1], sd))
+    ratio = cold_steps[both] / np.maximum(warm_steps[both], 1)
+    speedup = cold_wall / (warm_wall + wall_pred)
+    sur.close()
+    line = {"metric": "time-to-1e-6 residual vs plain FP32 GPU (learned warm start)",
+            "value": round(speedup, 3), "unit": "x", "higher_is_better": True, "dtype": "f32 solver, bf16 surrogate",
+            "data": "synthetic seeded obstacle channels",
+            "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training / "
+                                   f"{args.ntest} test channels, {args.epochs} epochs"},
+            "cold": {"wall_s": round(cold_wall, 3), "steps_median": float(np.median(cold_steps[both]))},
+            "warm": {"wall_s": round(warm_wall, 3), "predict_s": round(wall_pred, 3),
+                     "steps_median": float(np.median(warm_steps[both]))},
+            "steps_ratio_median": float(np.median(ratio)), "converged_cases": int(both.sum()),
+            "u_rel_l2_vs_fp64": {"cold_max": max(e_cold), "warm_max": max(e_warm), "prediction_median":
+                                 float(np.median(e_pred))},
+            "training": {"fields_s": round(wall_gen, 1), "train_s": round(wall_train, 1),
+                         "loss_first": losses[0], "loss_last": losses[-1]},
+            "total_s": round(time.perf_counter() - t0, 1)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,8 +242,11 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "batch", "sphere"],
-                    help="tgv: the headline 256^3 box (default); batch / sphere: time-to-residual workloads")
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "batch", "sphere", "warmstart"],
+                    help="tgv: the headline 256^3 box (default); batch / sphere / warmstart: time-to-residual workloads")
+    ap.add_argument("--ntrain", type=int, default=1024)
+    ap.add_argument("--ntest", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--ndom", type=int, default=2048)
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--forms", default="f32,f16")
@@ -206,6 +268,8 @@ def main():
         return run_batch(args)
     if args.workload == "sphere":
         return run_sphere(args)
+    if args.workload == "warmstart":
+        return run_warmstart(args)
 
     import numpy as np
 

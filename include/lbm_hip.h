@@ -151,6 +151,14 @@ int lbm_reduce(lbm_handle *h, lbm_sums *out);
 int lbm_get_populations(lbm_handle *h, double *f);
 int lbm_set_populations(lbm_handle *h, const double *f);
 
+/* Change the population storage (LBM_STORE_*) keeping the state: the
+ * populations are converted on the device (exact when widening, round to
+ * nearest even when narrowing).  A residual check needs a snapshot step
+ * after the change (lbm_step_check and LBM_STEP_SNAPSHOT take one).  No
+ * reference counterpart: used to run the bulk of a solve in 16-bit storage
+ * and finish in FP32. */
+int lbm_set_storage(lbm_handle *h, int32_t storage);
+
 /* Time nsteps steps on the handle's stream with HIP events (ms total). */
 int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms);
 

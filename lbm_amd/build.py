@@ -25,9 +25,11 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
-# translation units of liblbm_hip.so: the API and set-up kernels, and the
-# step kernels of each storage type (compiled in parallel)
-HIP_SOURCES = ["lbm_hip.hip", "lbm_step_f32.hip", "lbm_step_f64.hip", "lbm_step_f16.hip", "lbm_step_bf16.hip"]
+# translation units of liblbm_hip.so: the API and set-up kernels, the step
+# kernels of each storage type and the warm-start network (compiled in
+# parallel)
+HIP_SOURCES = ["lbm_hip.hip", "lbm_step_f32.hip", "lbm_step_f64.hip", "lbm_step_f16.hip", "lbm_step_bf16.hip",
+               "lbm_surrogate.hip"]
 
 
 def build_hip(extra_flags=(), out=None, jobs=None):

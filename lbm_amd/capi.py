@@ -49,7 +49,7 @@ class Sums(ct.Structure):
 # every symbol declared in include/lbm_hip.h
 EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_get_macros",
            "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_set_launch",
-           "lbm_set_index_width", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
+           "lbm_set_index_width", "lbm_set_storage", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
 _lib = None
 
@@ -77,6 +77,7 @@ def load():
     lib.lbm_time_steps.argtypes = [P, ct.c_int32, dp]
     lib.lbm_set_launch.argtypes = [P, ct.c_int32, ct.c_int32, ct.c_int32]
     lib.lbm_set_index_width.argtypes = [P, ct.c_int32]
+    lib.lbm_set_storage.argtypes = [P, ct.c_int32]
     lib.lbm_info.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]
     lib.lbm_synchronize.argtypes = [P]
     lib.lbm_destroy.argtypes = [P]
@@ -207,6 +208,11 @@ class Lattice:
 
     def set_index_width(self, bits):
         self._check(self.lib.lbm_set_index_width(self.h, int(bits)))
+
+    def set_storage(self, storage):
+        """Convert the populations to another storage form on the device."""
+        self._check(self.lib.lbm_set_storage(self.h, STORE[storage]))
+        self.storage = storage
 
     def sync(self):
         self._check(self.lib.lbm_synchronize(self.h))

@@ -681,6 +681,53 @@ int lbm_set_populations(lbm_handle *h, const double *f) {
   return LBM_OK;
 }
 
+int lbm_set_storage(lbm_handle *h, int32_t storage) {
+  if (!h) return fail(h, LBM_EINVAL, "lbm_set_storage: null handle");
+  const int es = esize_of(storage);
+  if (!es) return fail(h, LBM_EINVAL, "lbm_set_storage: unknown storage");
+  if (storage == h->storage) return LBM_OK;
+  if ((unsigned long long)h->plane * h->Q * es > (unsigned long long)h->plane * h->Q * h->esize) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+        (unsigned long long)free_b < (unsigned long long)h->plane * h->Q * (2ull * es) + h->ncell * h->Q * 8ull)
+      return fail(h, LBM_ENOMEM, "lbm_set_storage: not enough device memory for the conversion");
+  }
+  // current state -> FP64 f (exact for every storage form) -> new storage
+  double *fd = nullptr;
+  HIPCHK(h, hipMalloc(&fd, (size_t)h->ncell * h->Q * sizeof(double)));
+  KArgs a = make_args(h);
+  dim3 g, b;
+  launch_dims(h, g, b);
+  dispatch_qs<GetL>(h->Q, h->storage, a, g, b, h->stream, fd);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(h, hipFree(h->buf[k]));
+    h->buf[k] = nullptr;
+    if (h->snap[k]) HIPCHK(h, hipFree(h->snap[k]));
+    h->snap[k] = nullptr;
+  }
+  const size_t bytes = (size_t)h->plane * h->Q * es;
+  for (int k = 0; k < 2; ++k) {
+    if (hipMalloc(&h->buf[k], bytes) != hipSuccess) {
+      (void)hipFree(fd);
+      return fail(h, LBM_ENOMEM, "lbm_set_storage: hipMalloc of populations failed");
+    }
+    HIPCHK(h, hipMemsetAsync(h->buf[k], 0, bytes, h->stream));
+  }
+  h->storage = storage;
+  h->esize = es;
+  h->w64 = (unsigned long long)h->plane * (unsigned long long)es > 0xffffffffull;
+  h->snap_last = 0;
+  a = make_args(h);
+  a.dst = h->buf[h->cur];
+  dispatch_qs<SetL>(h->Q, h->storage, a, g, b, h->stream, (const double *)fd);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipFree(fd));
+  return ensure_snapshots(h);
+}
+
 int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms) {
   if (!h || !ms || nsteps <= 0) return fail(h, LBM_EINVAL, "lbm_time_steps: bad argument");
   hipEvent_t e0, e1;

@@ -1,0 +1,96 @@
+"""Learned warm start for steady 2D obstacle channels (256 x 128 D2Q9).
+
+Pipeline: seeded obstacle channels -> steady fields from the batched solver
+(include/lbm_batch.h) -> features/targets -> surrogate training on the GPU
+(include/lbm_surrogate.h) -> prediction used as the initial field of new
+channels, compared with the plain FP32 start (rho = 1, inlet velocity in
+every fluid cell) on wall time and steps to the relative residual 1e-6, and
+on the velocity error against FP64 solutions converged to 1e-10.
+"""
+import time
+
+import numpy as np
+
+from . import capi
+from . import surrogate as S
+from . import workloads as W
+
+U_IN = 0.05
+NX, NY = 256, 128
+
+
+def make_set(n, seed):
+    rng = np.random.default_rng(seed)
+    doms, feats = [], []
+    for _ in range(n):
+        dom, re, _ = W.obstacle_channel(rng, NX, NY, U_IN)
+        solid = dom["solid"].reshape(NY, NX).astype(bool)
+        doms.append(dom)
+        feats.append(S.features(solid, re))
+    return doms, np.stack(feats)
+
+
+def _init(b, doms, fields=None):
+    if fields is None:
+        rho, u = zip(*[W.plug_start(d, U_IN) for d in doms])
+    else:
+        rho, u = zip(*fields)
+    b.init(np.concatenate(rho), np.concatenate(u))
+
+
+def solve(doms, tol, max_steps, storage="f32", fields=None, interval=100, tiling=None):
+    """Run a batch to tol; returns (steps[n], res[n], wall seconds, [(rho, u)])."""
+    b = capi.Batch(doms, storage=storage)
+    if tiling:
+        b.set_tiling(*tiling)
+    _init(b, doms, fields)
+    t0 = time.perf_counter()
+    steps, res, _, ms = b.run(interval, tol, max_steps)
+    wall = time.perf_counter() - t0
+    out = [b.macros(k) for k in range(len(doms))]
+    b.close()
+    return steps, res, wall, ms / 1e3, out
+
+
+def targets(doms, fields):
+    t = np.stack([S.encode_targets(r, u, U_IN).reshape(NY, NX, 3) for r, u in fields])
+    m = np.stack([(d["solid"].reshape(NY, NX) == 0) for d in doms]).astype(np.uint8)
+    return t, m
+
+
+def train(sur, x, t, m, epochs, lr0, lr1, seed=0, log=None):
+    """Adam with a cosine learning-rate schedule from lr0 to lr1."""
+    sur.set_dataset(x, t, m)
+    n = x.shape[0]
+    rng = np.random.default_rng(seed)
+    steps_per_epoch = n // sur.batch
+    total = epochs * steps_per_epoch
+    losses = []
+    k = 0
+    t0 = time.perf_counter()
+    for e in range(epochs):
+        perm = rng.permutation(n)[:steps_per_epoch * sur.batch]
+        # one call per epoch with a fixed learning rate for that epoch
+        lr = lr1 + 0.5 * (lr0 - lr1) * (1 + np.cos(np.pi * k / max(1, total)))
+        l = sur.train(perm, lr)
+        k += steps_per_epoch
+        losses.append(float(np.mean(l)))
+        if log:
+            log(f"# epoch {e + 1}/{epochs} lr {lr:.2e} loss {losses[-1]:.3e} ({time.perf_counter() - t0:.1f} s)")
+    return losses, time.perf_counter() - t0
+
+
+def predict_fields(sur, x, doms):
+    out = []
+    t0 = time.perf_counter()
+    for i in range(0, x.shape[0], sur.batch):
+        y = sur.forward(x[i:i + sur.batch])
+        for k in range(y.shape[0]):
+            d = doms[i + k]
+            out.append(S.decode_prediction(y[k], d["solid"], U_IN))
+    return out, time.perf_counter() - t0
+
+
+def rel_l2(u, ref, solid):
+    f = ~np.asarray(solid, bool).reshape(-1)
+    return float(np.linalg.norm(u[f] - ref[f]) / np.linalg.norm(ref[f]))

@@ -74,8 +74,19 @@ def plug_start(dom, u_in):
     return rho, u
 
 
+def neem_channel_faces(u_in, rho_out=1.0):
+    """NEEM velocity inlet and NEEM pressure outlet with bounce-back walls: the
+    open faces that stay stable and converge down to tau ~ 0.54 (the Zou-He
+    inlet and the anti-bounce-back outlet do not)."""
+    f = dict(WALLS2)
+    f["-x"] = ("velocity_neem", (u_in, 0.0), 1.0)
+    f["+x"] = ("pressure_neem", None, rho_out)
+    return f
+
+
 def obstacle_channel(rng, nx=256, ny=128, u_in=0.05):
-    """One seeded obstacle channel; returns (domain dict, Re, obstacle list)."""
+    """One seeded obstacle channel (Re on the channel height, NEEM inlet and
+    outlet, bounce-back walls); returns (domain dict, Re, obstacle list)."""
     re = float(rng.uniform(20.0, 200.0))
     nu = u_in * ny / re
     tau = 3.0 * nu + 0.5
@@ -96,5 +107,5 @@ def obstacle_channel(rng, nx=256, ny=128, u_in=0.05):
             obs.append(("rect", cx, cy, r, ry))
         solid |= m
     dom = {"shape": (nx, ny), "omega": 1.0 / tau, "solid": solid.astype(np.uint8).reshape(-1),
-           "faces": channel_faces(u_in), "tau": tau, "re": re}
+           "faces": neem_channel_faces(u_in), "tau": tau, "re": re}
     return dom, re, obs

@@ -1,5 +1,5 @@
-"""CPU-side checks of the C ABI: the in-tree library loads and exports every
-entry point that include/lbm_hip.h declares (no compute calls without a GPU)."""
+"""CPU-side checks of the C ABI: the in-tree libraries load and export every
+entry point that include/*.h declares (no compute calls without a GPU)."""
 import ctypes as ct
 import os
 import re
@@ -15,14 +15,16 @@ def declared(header):
     return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", src)))
 
 
-@pytest.mark.parametrize("header,lib", [("lbm_hip.h", "lbm_amd/liblbm_hip.so")])
+@pytest.mark.parametrize("header,lib", [("lbm_hip.h", "lbm_amd/liblbm_hip.so"), ("lbm_batch.h", "lbm_amd/liblbm_hip.so"),
+                                        ("lbm_surrogate.h", "lbm_amd/liblbm_hip.so"),
+                                        ("lbm_host.h", "lbm_amd/liblbm_host.so")])
 def test_library_exports_header(header, lib):
     path = os.path.join(ROOT, lib)
     if not os.path.exists(path):
         pytest.fail(f"{path} not built")
     so = ct.CDLL(path)
     names = declared(header)
-    assert len(names) >= 10
+    assert len(names) >= 5
     for n in names:
         assert hasattr(so, n), n
 
@@ -31,3 +33,15 @@ def test_python_binding_lists_all_exports():
     from lbm_amd import capi
 
     assert sorted(capi.EXPORTS) == declared("lbm_hip.h")
+
+
+def test_python_bindings_list_batch_and_surrogate_exports():
+    from lbm_amd import capi, surrogate
+
+    assert sorted(capi.BATCH_EXPORTS) == declared("lbm_batch.h")
+    assert sorted(surrogate.EXPORTS) == declared("lbm_surrogate.h")
+
+
+def test_every_header_is_checked():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["lbm_batch.h", "lbm_hip.h", "lbm_host.h", "lbm_surrogate.h"]

@@ -298,3 +298,46 @@ def test_neem_next_to_open_face_rejected():
         lat.set_faces({"-x": ("velocity_zh", (0.02, 0.0), 1.0), "+x": ("pressure_neem", None, 1.0),
                        "-y": ("pressure_abb", None, 1.0), "+y": ("wall", None, 1.0)})
     lat.close()
+
+
+def test_set_storage_conversions_exact():
+    """lbm_set_storage converts on the device: widening is exact (FP32 ->
+    FP64 -> FP32 returns the same bits), narrowing rounds to nearest even
+    (FP64 -> FP32 and FP32 -> FP16 match numpy's rounding of h = f - w), and
+    stepping after the conversion equals a fresh handle of the new storage
+    loaded with the same populations."""
+    shape = (20, 12, 10)
+    solid = sphere_solid(shape, (8, 6, 5), 2.5)
+    N = int(np.prod(shape))
+    rng = np.random.default_rng(21)
+    rho0, u0 = 1 + 0.01 * rng.standard_normal(N), 0.02 * rng.standard_normal((N, 3))
+    faces = channel_faces3()
+    faces["+x"] = ("pressure_neem", None, 1.0)
+    _, w, _ = O.tables(19)
+    w = w[:, None]
+    lat = capi.Lattice(shape, storage="f64", omega=1 / 0.6, faces=faces, solid=solid)
+    lat.init(rho0, u0)
+    lat.step(5)
+    p64 = lat.populations()
+    print("set_storage f64 -> f32", flush=True)
+    lat.set_storage("f32")
+    p32 = lat.populations()
+    assert np.array_equal(p32, (p64 - w).astype(np.float32).astype(np.float64) + w)
+    lat.set_storage("f64")
+    assert np.array_equal(lat.populations(), p32)
+    lat.set_storage("f32")
+    assert np.array_equal(lat.populations(), p32)
+    print("set_storage f32 -> f16", flush=True)
+    lat.set_storage("f16")
+    p16 = lat.populations()
+    assert np.array_equal(p16, (p32 - w).astype(np.float32).astype(np.float16).astype(np.float64) + w)
+    # stepping continues exactly like a fresh FP16 handle holding the same state
+    fresh = capi.Lattice(shape, storage="f16", omega=1 / 0.6, faces=faces, solid=solid)
+    fresh.set_populations(p16)
+    lat.step(6)
+    fresh.step(6)
+    assert np.array_equal(lat.populations(), fresh.populations())
+    s, rc = lat.step_check()
+    assert rc == 0 and np.isfinite(s.res_rel)
+    fresh.close()
+    lat.close()
