@@ -193,41 +193,67 @@ def run_warmstart(args):
         print(msg, flush=True)
 
     t0 = time.perf_counter()
-    doms, x = WS.make_set(args.ntrain, seed=11)
-    steps, res, wall_gen, _, fields = WS.solve(doms, 1e-5, args.max_steps)
-    ok = steps > 0
-    log(f"# training set: {int(ok.sum())}/{len(doms)} channels converged to 1e-5 in {wall_gen:.1f} s "
-        f"(steps {int(steps[ok].min())}..{int(steps[ok].max())})")
-    keep = np.nonzero(ok)[0]
-    t, m = WS.targets([doms[i] for i in keep], [fields[i] for i in keep])
     sur = S.Surrogate(batch=16, height=WS.NY, width=WS.NX, seed=7)
-    losses, wall_train = WS.train(sur, x[keep], t, m, args.epochs, 1e-3, 2e-5, log=log)
+    if args.load_model:
+        norm = WS.load_model(sur, args.load_model)
+        wall_gen = wall_train = 0.0
+        losses = [float("nan")]
+    else:
+        doms, x = WS.make_set(args.ntrain, seed=11)
+        steps, res, wall_gen, _, fields = WS.solve(doms, 1e-5, args.max_steps)
+        ok = steps > 0
+        log(f"# training set: {int(ok.sum())}/{len(doms)} channels converged to 1e-5 in {wall_gen:.1f} s "
+            f"(steps {int(steps[ok].min())}..{int(steps[ok].max())})")
+        keep = np.nonzero(ok)[0]
+        t, m, norm = WS.targets([doms[i] for i in keep], [fields[i] for i in keep])
+        del fields
+        losses, wall_train = WS.train(sur, x[keep], t, m, args.epochs, args.lr, args.lr / 50, log=log)
+        del x, t, m
+        if args.save_model:
+            WS.save_model(sur, norm, args.save_model)
     tdoms, tx = WS.make_set(args.ntest, seed=12)
-    cold_steps, _, cold_wall, cold_dev, cold_fields = WS.solve(tdoms, 1e-6, args.max_steps)
-    pred, wall_pred = WS.predict_fields(sur, tx, tdoms)
-    warm_steps, _, warm_wall, warm_dev, warm_fields = WS.solve(tdoms, 1e-6, args.max_steps, fields=pred)
-    ref_steps, _, _, _, ref_fields = WS.solve(tdoms, 1e-10, 4 * args.max_steps, storage="f64", fields=cold_fields)
-    both = (cold_steps > 0) & (warm_steps > 0) & (ref_steps > 0)
-    e_cold, e_warm, e_pred = [], [], []
-    for i in np.nonzero(both)[0]:
-        sd = tdoms[i]["solid"]
-        e_cold.append(WS.rel_l2(cold_fields[i][1], ref_fields[i][1], sd))
-        e_warm.append(WS.rel_l2(warm_fields[i][1], ref_fields[i][1], sd))
-        e_pred.append(WS.rel_l2(pred[i][1], ref_fields[i][1], sd))
-    ratio = cold_steps[both] / np.maximum(warm_steps[both], 1)
-    speedup = cold_wall / (warm_wall + wall_pred)
+    pred, wall_pred = WS.predict_fields(sur, tx, tdoms, norm)
+    pred_per_case = wall_pred / len(tdoms)
     sur.close()
+    # each unseen channel solved alone (a batch of one domain: graph-captured
+    # cycles of 100 steps), as a user would solve one problem
+    res = {"cold": [], "warm": [], "two": []}
+    for i, d in enumerate(tdoms):
+        sc, _, wc, _, fc = WS.solve([d], 1e-6, args.max_steps)
+        sw, _, ww, _, fw = WS.solve([d], 1e-6, args.max_steps, fields=[pred[i]])
+        s1, s2, w2, f2 = WS.solve_two_phase([d], [pred[i]], args.tol16, 1e-6, args.max_steps)
+        res["cold"].append((int(sc[0]), wc, fc[0]))
+        res["warm"].append((int(sw[0]), ww + pred_per_case, fw[0]))
+        res["two"].append((int(s1[0]) + int(s2[0]) if s1[0] > 0 and s2[0] > 0 else -1, w2 + pred_per_case, f2[0]))
+    ref_steps, _, _, _, ref_fields = WS.solve(tdoms, 1e-10, 4 * args.max_steps, storage="f64",
+                                              fields=[r[2] for r in res["cold"]])
+    ok = np.array([res["cold"][i][0] > 0 and res["warm"][i][0] > 0 and res["two"][i][0] > 0 and ref_steps[i] > 0
+                   for i in range(len(tdoms))])
+    idx = np.nonzero(ok)[0]
+
+    def stats(key):
+        sp = np.array([res["cold"][i][1] / res[key][i][1] for i in idx])
+        st = np.array([res["cold"][i][0] / max(1, res[key][i][0]) for i in idx])
+        err = [WS.rel_l2(res[key][i][2][1], ref_fields[i][1], tdoms[i]["solid"]) for i in idx]
+        return {"speedup_median": round(float(np.median(sp)), 3),
+                "speedup_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3),
+                "steps_ratio_median": round(float(np.median(st)), 3),
+                "wall_s_median": round(float(np.median([res[key][i][1] for i in idx])), 4),
+                "steps_median": float(np.median([res[key][i][0] for i in idx])),
+                "u_rel_l2_vs_fp64_max": float(max(err)), "u_rel_l2_vs_fp64_median": float(np.median(err))}
+
+    cold = stats("cold")
+    warm = stats("warm")
+    two = stats("two")
+    e_pred = [WS.rel_l2(pred[i][1], ref_fields[i][1], tdoms[i]["solid"]) for i in idx]
     line = {"metric": "time-to-1e-6 residual vs plain FP32 GPU (learned warm start)",
-            "value": round(speedup, 3), "unit": "x", "higher_is_better": True, "dtype": "f32 solver, bf16 surrogate",
-            "data": "synthetic seeded obstacle channels",
+            "value": warm["speedup_median"], "unit": "x", "higher_is_better": True,
+            "dtype": "f32 solver, bf16 surrogate", "data": "synthetic seeded obstacle channels",
             "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training / "
-                                   f"{args.ntest} test channels, {args.epochs} epochs"},
-            "cold": {"wall_s": round(cold_wall, 3), "steps_median": float(np.median(cold_steps[both]))},
-            "warm": {"wall_s": round(warm_wall, 3), "predict_s": round(wall_pred, 3),
-                     "steps_median": float(np.median(warm_steps[both]))},
-            "steps_ratio_median": float(np.median(ratio)), "converged_cases": int(both.sum()),
-            "u_rel_l2_vs_fp64": {"cold_max": max(e_cold), "warm_max": max(e_warm), "prediction_median":
-                                 float(np.median(e_pred))},
+                                   f"{args.ntest} test channels, {args.epochs} epochs, each test channel solved alone"},
+            "cold_f32": cold, "warm_f32": warm, "warm_f16_then_f32": dict(two, tol_f16=args.tol16),
+            "converged_cases": int(ok.sum()), "prediction_u_rel_l2_median": float(np.median(e_pred)),
+            "predict_s_per_case": round(pred_per_case, 4),
             "training": {"fields_s": round(wall_gen, 1), "train_s": round(wall_train, 1),
                          "loss_first": losses[0], "loss_last": losses[-1]},
             "total_s": round(time.perf_counter() - t0, 1)}
@@ -247,6 +273,11 @@ def main():
     ap.add_argument("--ntrain", type=int, default=1024)
     ap.add_argument("--ntest", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--tol16", type=float, default=1e-3,
+                    help="warm start: residual at which FP16 storage hands over to FP32 (its rounding floor is ~5e-4)")
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--save-model", default=None)
+    ap.add_argument("--load-model", default=None)
     ap.add_argument("--ndom", type=int, default=2048)
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--forms", default="f32,f16")

@@ -4,7 +4,7 @@
  *
  * No reference counterpart: SFCMM/LBM always starts from rest
  * (src/lbm/solver.cpp:267-295).  A convolutional encoder-decoder (U-Net,
- * 15 3x3 convolutions on 4 resolution levels, about 2M parameters) maps
+ * 23 3x3 convolutions on 6 resolution levels, about 1.9M parameters) maps
  * per-cell geometry features to the steady (u_x, u_y, rho) field of a 2D
  * channel; its prediction initialises the solver (lbm_init_fields), which
  * then converges as usual.  Every kernel is hand-written HIP for gfx950:
@@ -13,7 +13,7 @@
  *
  * Layouts (host arrays): features x[n][H][W][in_channels] FP32, targets
  * t[n][H][W][3] FP32, fluid masks m[n][H][W] (1 = cell enters the loss).
- * H and W must be multiples of 16, in_channels <= 16.  Loss: mean over
+ * H and W must be multiples of 32, in_channels <= 16.  Loss: mean over
  * fluid cells and the 3 outputs of the squared error.  Status codes as in
  * lbm_hip.h.
  */

@@ -1,5 +1,6 @@
 // Learned warm start (include/lbm_surrogate.h): a U-Net of 3x3
-// convolutions trained and evaluated with hand-written gfx950 kernels.
+// convolutions on six resolution levels, trained and evaluated with
+// hand-written gfx950 kernels.
 //
 // Tensors are NHWC.  Activations are bf16 (uint16 bit patterns), gradients
 // of activations FP32, parameters FP32 masters with bf16 working copies.
@@ -443,8 +444,13 @@ static int add_tensor(lbm_sur *s, int level, int c, bool relu) {
   return (int)s->t.size() - 1;
 }
 
-// network: encoder 32-64-128-256 (two convs per level, 2x2 average pooling),
-// decoder with nearest upsampling and skip concatenation, 3 linear outputs
+// network: six resolution levels (widths 32, 32, 64, 96, 128, 160; the
+// coarsest level is 1/32 of the input, so the receptive field spans the
+// whole channel), two 3x3 convolutions per level, 2x2 average pooling down,
+// nearest upsampling and skip concatenation up, 3 linear outputs
+constexpr int NLEV = 6;
+constexpr int WIDTH[NLEV] = {32, 32, 64, 96, 128, 160};
+
 static void build_net(lbm_sur *s) {
   const int x0 = add_tensor(s, 0, 16, false);
   auto conv = [&](int a, int sidx, int co, bool relu, bool last = false) {
@@ -465,14 +471,15 @@ static void build_net(lbm_sur *s) {
     s->pool_pairs.push_back({src, dst});
     return dst;
   };
-  int e1 = conv(-1, conv(-1, x0, 32, true), 32, true);
-  int e2 = conv(-1, conv(-1, pool(e1), 64, true), 64, true);
-  int e3 = conv(-1, conv(-1, pool(e2), 128, true), 128, true);
-  int bn = conv(-1, conv(-1, pool(e3), 256, true), 256, true);
-  int d3 = conv(-1, conv(bn, e3, 128, true), 128, true);
-  int d2 = conv(-1, conv(d3, e2, 64, true), 64, true);
-  int d1 = conv(-1, conv(d2, e1, 32, true), 32, true);
-  conv(-1, d1, 32, false, true);  // 3 used outputs, padded to one 32-channel tile
+  int enc[NLEV];
+  int cur = x0;
+  for (int lv = 0; lv < NLEV; ++lv) {
+    if (lv > 0) cur = pool(cur);
+    cur = conv(-1, conv(-1, cur, WIDTH[lv], true), WIDTH[lv], true);
+    enc[lv] = cur;
+  }
+  for (int lv = NLEV - 2; lv >= 0; --lv) cur = conv(-1, conv(cur, enc[lv], WIDTH[lv], true), WIDTH[lv], true);
+  conv(-1, cur, 32, false, true);  // 3 used outputs, padded to one 32-channel tile
   long long off = 0;
   for (Layer &l : s->L) {
     l.woff = off;
@@ -670,9 +677,9 @@ static int check_n(lbm_sur *s, int n) {
 extern "C" {
 
 int lbm_sur_create(int32_t batch, int32_t height, int32_t width, int32_t in_channels, uint64_t seed, lbm_sur **out) {
-  if (!out || batch < 1 || height < 16 || width < 16 || height % 16 || width % 16 || in_channels < 1 ||
+  if (!out || batch < 1 || height < 32 || width < 32 || height % 32 || width % 32 || in_channels < 1 ||
       in_channels > 16)
-    return sfail(nullptr, LBM_EINVAL, "lbm_sur_create: batch >= 1, height and width multiples of 16, in_channels 1..16");
+    return sfail(nullptr, LBM_EINVAL, "lbm_sur_create: batch >= 1, height and width multiples of 32, in_channels 1..16");
   lbm_sur *s = new lbm_sur();
   s->B = batch;
   s->H = height;

@@ -53,9 +53,14 @@ def solve(doms, tol, max_steps, storage="f32", fields=None, interval=100, tiling
 
 
 def targets(doms, fields):
+    """Encoded targets [n][NY][NX][3], fluid masks, and the per-channel
+    (mean, std) over fluid cells used to standardise them."""
     t = np.stack([S.encode_targets(r, u, U_IN).reshape(NY, NX, 3) for r, u in fields])
     m = np.stack([(d["solid"].reshape(NY, NX) == 0) for d in doms]).astype(np.uint8)
-    return t, m
+    fl = m.astype(bool)
+    mu = np.array([t[..., c][fl].mean() for c in range(3)], np.float32)
+    sd = np.array([t[..., c][fl].std() for c in range(3)], np.float32)
+    return ((t - mu) / sd).astype(np.float32), m, (mu, sd)
 
 
 def train(sur, x, t, m, epochs, lr0, lr1, seed=0, log=None):
@@ -80,11 +85,12 @@ def train(sur, x, t, m, epochs, lr0, lr1, seed=0, log=None):
     return losses, time.perf_counter() - t0
 
 
-def predict_fields(sur, x, doms):
+def predict_fields(sur, x, doms, norm):
+    mu, sd = norm
     out = []
     t0 = time.perf_counter()
     for i in range(0, x.shape[0], sur.batch):
-        y = sur.forward(x[i:i + sur.batch])
+        y = sur.forward(x[i:i + sur.batch]) * sd + mu
         for k in range(y.shape[0]):
             d = doms[i + k]
             out.append(S.decode_prediction(y[k], d["solid"], U_IN))
@@ -94,3 +100,22 @@ def predict_fields(sur, x, doms):
 def rel_l2(u, ref, solid):
     f = ~np.asarray(solid, bool).reshape(-1)
     return float(np.linalg.norm(u[f] - ref[f]) / np.linalg.norm(ref[f]))
+
+
+def solve_two_phase(doms, fields, tol1, tol, max_steps, storage1="f16", interval=100):
+    """Reduced storage first (to tol1, above its rounding floor), then FP32 from
+    the phase-1 fields to tol.  Returns (steps1, steps2, wall seconds, fields)."""
+    s1, _, w1, _, f1 = solve(doms, tol1, max_steps, storage=storage1, fields=fields, interval=interval)
+    s2, _, w2, _, f2 = solve(doms, tol, max_steps, storage="f32", fields=f1, interval=interval)
+    return s1, s2, w1 + w2, f2
+
+
+def save_model(sur, norm, path):
+    mu, sd = norm
+    np.savez(path, params=sur.params(), mu=mu, sd=sd)
+
+
+def load_model(sur, path):
+    z = np.load(path)
+    sur.set_params(z["params"])
+    return z["mu"], z["sd"]

@@ -17,10 +17,8 @@ fluid cells and the 3 outputs.
 """
 import numpy as np
 
-# (a, s, co, relu): a = index of the tensor upsampled into the concatenation,
-# s = index of the same-resolution input; tensors: 0 = input (16 channels),
-# then every layer output and pooled tensor in creation order.
-ARCH = None
+# widths of the six resolution levels
+WIDTHS = (32, 32, 64, 96, 128, 160)
 
 
 def build():
@@ -44,14 +42,16 @@ def build():
         pools.append((src, len(tensors) - 1))
         return len(tensors) - 1
 
-    e1 = conv(None, conv(None, 0, 32, True), 32, True)
-    e2 = conv(None, conv(None, pool(e1), 64, True), 64, True)
-    e3 = conv(None, conv(None, pool(e2), 128, True), 128, True)
-    bn = conv(None, conv(None, pool(e3), 256, True), 256, True)
-    d3 = conv(None, conv(bn, e3, 128, True), 128, True)
-    d2 = conv(None, conv(d3, e2, 64, True), 64, True)
-    d1 = conv(None, conv(d2, e1, 32, True), 32, True)
-    conv(None, d1, 32, False, last=True)
+    enc = []
+    cur = 0
+    for lv, wdt in enumerate(WIDTHS):
+        if lv > 0:
+            cur = pool(cur)
+        cur = conv(None, conv(None, cur, wdt, True), wdt, True)
+        enc.append(cur)
+    for lv in range(len(WIDTHS) - 2, -1, -1):
+        cur = conv(None, conv(cur, enc[lv], WIDTHS[lv], True), WIDTHS[lv], True)
+    conv(None, cur, 32, False, last=True)
     off = 0
     for l in layers:
         ci = l["ca"] + l["cs"]
