@@ -397,13 +397,26 @@ class GridProblem(Problem):
                     r.u = (ct.c_double * 3)(*v)
                     self.init_u[int(cidx)] = v[:dim]
                     recs.append(r)
-                elif t == "pressure" and spec.get("model", "antibounceback") != "zouhe":
+                elif t == "pressure" and spec.get("model", "antibounceback") == "antibounceback":
                     r.kind = 2
                     r.rho = spec["pressure"]
                     inside = next(2 * a + (1 if nrm[a] < 0 else 0) for a in range(dim) if nrm[a] != 0)
                     r.n1 = int(nbr_table[cidx, inside])
                     r.n2 = int(nbr_table[r.n1, inside])
                     recs.append(r)
+                elif t in ("dirichlet", "pressure") and spec.get("model") == "neem":
+                    # every population from the inward neighbour; after all
+                    # bounce-back records, with the Zou-He records
+                    r.kind = 6 if t == "dirichlet" else 5
+                    inside = next(2 * a + (1 if nrm[a] < 0 else 0) for a in range(dim) if nrm[a] != 0)
+                    r.n1 = int(nbr_table[cidx, inside])
+                    if t == "dirichlet":
+                        v = list(spec["value"])[:dim] + [0.0] * (3 - dim)
+                        r.u = (ct.c_double * 3)(*v)
+                        self.init_u[int(cidx)] = v[:dim]
+                    else:
+                        r.rho = spec["pressure"]
+                    zh.append(r)
                 elif (t == "dirichlet" and spec["model"] == "zouhe") or (t == "pressure" and spec.get("model") == "zouhe"):
                     r.kind = 3 if t == "dirichlet" else 4
                     r.axis = d // 2

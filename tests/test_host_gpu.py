@@ -130,6 +130,8 @@ CASES = [
     ("pressure_channel2d.json", OWNCFG, 400, False),
     ("box3d_periodic_x.json", OWNCFG, 200, False),
     ("sphere3d_channel.json", OWNCFG, 150, True),
+    ("neem_channel2d.json", OWNCFG, 400, False),
+    ("sphere3d_neem.json", OWNCFG, 150, True),
 ]
 
 
