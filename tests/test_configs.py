@@ -53,11 +53,11 @@ def test_config1_oracle_against_startup_series():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 5e-4)])
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 1e-5)])
 def test_config1_gpu(storage, tol):
-    """Config 1 is an FP64 case.  FP32 storage is run for the record: the Guo
-    increments (~3e-7 per step on populations ~1e-2) sit a few hundred ulps
-    above FP32 resolution, so 20000 steps accumulate ~2e-4 relative error."""
+    """Config 1 is an FP64 case; FP32 storage (FP64 arithmetic) is held to the
+    FP32 parity bar as well: the Guo increments (~3e-7 per step) are applied
+    in FP64 and only the stored populations round (1.5e-6 after 20000 steps)."""
     from lbm_amd import capi
 
     N = NX * NY
