@@ -207,7 +207,9 @@ def run_warmstart(args):
         keep = np.nonzero(ok)[0]
         t, m, norm = WS.targets([doms[i] for i in keep], [fields[i] for i in keep])
         del fields
-        losses, wall_train = WS.train(sur, x[keep], t, m, args.epochs, args.lr, args.lr / 50, log=log)
+        xa, ta, ma = WS.mirror(x[keep], t, m) if args.mirror else (x[keep], t, m)
+        losses, wall_train = WS.train(sur, xa, ta, ma, args.epochs, args.lr, args.lr / 50, log=log)
+        del xa, ta, ma
         del x, t, m
         if args.save_model:
             WS.save_model(sur, norm, args.save_model)
@@ -249,8 +251,9 @@ def run_warmstart(args):
     line = {"metric": "time-to-1e-6 residual vs plain FP32 GPU (learned warm start)",
             "value": warm["speedup_median"], "unit": "x", "higher_is_better": True,
             "dtype": "f32 solver, bf16 surrogate", "data": "synthetic seeded obstacle channels",
-            "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training / "
-                                   f"{args.ntest} test channels, {args.epochs} epochs, each test channel solved alone"},
+            "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training "
+                                   f"({'+ mirrored, ' if args.mirror else ''}{args.epochs} epochs) / {args.ntest} test "
+                                   f"channels, each test channel solved alone"},
             "cold_f32": cold, "warm_f32": warm, "warm_f16_then_f32": dict(two, tol_f16=args.tol16),
             "converged_cases": int(ok.sum()), "prediction_u_rel_l2_median": float(np.median(e_pred)),
             "predict_s_per_case": round(pred_per_case, 4),
@@ -276,6 +279,7 @@ def main():
     ap.add_argument("--tol16", type=float, default=1e-3,
                     help="warm start: residual at which FP16 storage hands over to FP32 (its rounding floor is ~5e-4)")
     ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--mirror", type=int, default=1, help="train on the mirrored channels too")
     ap.add_argument("--save-model", default=None)
     ap.add_argument("--load-model", default=None)
     ap.add_argument("--ndom", type=int, default=2048)

@@ -249,6 +249,9 @@ __device__ void cell_velocity(const ST *src, long long plane, long long cell, do
 // cells, bounced back where one is missing (walls with the Ladd term,
 // solids plain; faces next to a NEEM face are walls or periodic, checked on
 // the host).  With a solid n1 the cell keeps its bounce-back values.
+template <int Q, typename ST, typename G>
+__device__ __forceinline__ void neem_correct(const G &a, int f, typename Store<ST>::R (&h)[Q]);
+
 template <int Q, typename ST, bool FLAGS, typename G>
 __device__ void neem_face(const G &a, int x, int y, int z, int f, typename Store<ST>::R (&h)[Q]) {
   using L = Lat<Q>;
@@ -294,6 +297,16 @@ __device__ void neem_face(const G &a, int x, int y, int z, int f, typename Store
                              (C(L::cx[j]) * C(a.face[face].u[0]) + C(L::cy[j]) * C(a.face[face].u[1]) +
                               C(L::cz[j]) * C(a.face[face].u[2])));
   }
+  neem_correct<Q, ST>(a, f, h);
+}
+
+// The NEEM rebuild of a face cell from its inward neighbour's post-streaming
+// populations, already in h.
+template <int Q, typename ST, typename G>
+__device__ __forceinline__ void neem_correct(const G &a, int f, typename Store<ST>::R (&h)[Q]) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  using R = typename Store<ST>::R;
   const Moments<Q, C> m = moments<Q, C>(h, C(a.F[0]), C(a.F[1]), C(a.F[2]));
   const bool vel = a.face[f].kind == LBM_FACE_VELOCITY_NEEM;
   const C rb = vel ? m.rho : C(a.face[f].rho);
@@ -333,6 +346,9 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
   const uint32_t lm = (FLAGS && active) ? a.links[cell] : 0u;
   if (lm & LINK_SELF_SOLID) active = false;
 
+  using R = typename Store<ST>::R;
+  R h[Q];
+  unsigned missing = 0;
   if (active) {
     // neighbour coordinates per axis with periodic wrap; v* = exists
     int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
@@ -345,9 +361,6 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
       if (z == 0) { if (a.per[2]) zm = a.nz - 1; else vzm = false; }
       if (z == a.nz - 1) { if (a.per[2]) zp = 0; else vzp = false; }
     }
-    using R = typename Store<ST>::R;
-    R h[Q];
-    unsigned missing = 0;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
@@ -372,9 +385,6 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
     // unwritten for a step.  Populations pulled from a solid cell (link
     // bits) are replaced below like the ones across a face.
     if (FLAGS) missing |= lm & ((1u << (Q - 1)) - 1u);
-
-    const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
-    const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
 
     if (missing) {
       // bounce-back source S(x, opp j) of every missing population
@@ -423,6 +433,41 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
         }
         // Zou-He faces: overwritten below
       }
+    }
+  }
+
+  // NEEM faces across x: the inward neighbour of a face cell is the adjacent
+  // lane of the same wave (a wave holds contiguous x of one row), so its
+  // post-streaming populations, complete after its own bounce-back, move
+  // across lanes instead of being gathered again.  The source lane is
+  // checked by its cell index; where it is not in the wave the face cell
+  // gathers (neem_face).
+  bool from_lane = false;
+  if (a.face[0].kind == LBM_FACE_PRESSURE_NEEM || a.face[0].kind == LBM_FACE_VELOCITY_NEEM ||
+      a.face[1].kind == LBM_FACE_PRESSURE_NEEM || a.face[1].kind == LBM_FACE_VELOCITY_NEEM) {
+    const bool lo = x == 0 && (a.face[0].kind == LBM_FACE_PRESSURE_NEEM || a.face[0].kind == LBM_FACE_VELOCITY_NEEM);
+    const bool hi = x == a.nx - 1 &&
+                    (a.face[1].kind == LBM_FACE_PRESSURE_NEEM || a.face[1].kind == LBM_FACE_VELOCITY_NEEM);
+    const bool face_lane = active && (lo || hi);
+    if (__ballot(face_lane)) {  // wave-uniform
+      const int lane = (int)__lane_id();
+      const int dir = lo ? 1 : -1;
+      const int sl = face_lane ? min(63, max(0, lane + dir)) : lane;
+      const unsigned key = active ? (unsigned)cell : 0xffffffffu;
+      const unsigned got = __shfl(key, sl, 64);
+      from_lane = face_lane && sl != lane && got == (unsigned)(dir > 0 ? cell + I(1) : cell - I(1));
+#pragma unroll
+      for (int j = 0; j < Q; ++j) {
+        const R t = __shfl(h[j], sl, 64);
+        if (from_lane) h[j] = t;
+      }
+    }
+  }
+
+  if (active) {
+    const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
+    const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
+    if (missing) {
       // Zou-He or NEEM on the first such face the cell lies on
 #pragma unroll 1
       for (int f = 0; f < 2 * L::D; ++f) {
@@ -434,8 +479,10 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
         const int coord = ax == 0 ? x : (ax == 1 ? y : z);
         const int n = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
         if (coord != ((f & 1) ? n - 1 : 0)) continue;
-        if (k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM)
-          neem_face<Q, ST, FLAGS>(a, x, y, z, f, h);
+        if (k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM) {
+          if (from_lane && ax == 0) neem_correct<Q, ST>(a, f, h);
+          else neem_face<Q, ST, FLAGS>(a, x, y, z, f, h);
+        }
         else
           zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
                        R(a.face[f].u[2]), R(a.face[f].rho));

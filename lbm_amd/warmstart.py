@@ -59,8 +59,22 @@ def targets(doms, fields):
     m = np.stack([(d["solid"].reshape(NY, NX) == 0) for d in doms]).astype(np.uint8)
     fl = m.astype(bool)
     mu = np.array([t[..., c][fl].mean() for c in range(3)], np.float32)
+    mu[1] = 0.0  # u_y: zero mean by the mirror symmetry of the channel family
     sd = np.array([t[..., c][fl].std() for c in range(3)], np.float32)
     return ((t - mu) / sd).astype(np.float32), m, (mu, sd)
+
+
+def mirror(x, t, m):
+    """Append the channels mirrored about the centre line (y -> H-1-y): the
+    flow of a mirrored geometry is the mirrored flow with u_y negated.  The
+    y feature and the u_y target change sign; both targets are standardised
+    with a mean that is zero for u_y by symmetry, so negation keeps them
+    standardised."""
+    xf = x[:, ::-1].copy()
+    xf[..., 1] = -xf[..., 1]
+    tf = t[:, ::-1].copy()
+    tf[..., 1] = -tf[..., 1]
+    return (np.concatenate([x, xf]), np.concatenate([t, tf]), np.concatenate([m, m[:, ::-1]]))
 
 
 def train(sur, x, t, m, epochs, lr0, lr1, seed=0, log=None):

@@ -18,11 +18,14 @@ pytestmark = pytest.mark.gpu
 def small_domains(seed=5):
     rng = np.random.default_rng(seed)
     doms = []
-    for k, (nx, ny) in enumerate([(37, 21), (64, 32), (70, 9), (128, 40), (16, 16), (93, 57)]):
+    for k, (nx, ny) in enumerate([(37, 21), (65, 32), (70, 9), (33, 40), (16, 16), (93, 57)]):
         solid = np.zeros((ny, nx), dtype=np.uint8)
         if k % 2 == 0:
             solid[: ny // 3, : nx // 4] = 1  # a backward-facing step
-        faces = W.channel_faces(0.03) if k != 4 else {}  # domain 4 fully periodic
+        # domain 4 fully periodic; 1 and 3 with NEEM inlet/outlet (their x
+        # faces exercise both the in-wave neighbour exchange and its
+        # gathering fallback, depending on where the face falls in a tile)
+        faces = {} if k == 4 else (W.neem_channel_faces(0.03) if k in (1, 3) else W.channel_faces(0.03))
         doms.append({"shape": (nx, ny), "omega": float(rng.uniform(1.0, 1.8)), "solid": solid.reshape(-1),
                      "faces": faces})
     return doms
@@ -38,12 +41,13 @@ def random_init(doms, seed=1):
     return rhos, us
 
 
-@pytest.mark.parametrize("storage", ["f32", "f64", "f16"])
-def test_batch_equals_single_domain_over_graph_replays(storage):
+@pytest.mark.parametrize("storage,tile", [("f32", 64), ("f64", 64), ("f16", 64), ("f32", 32), ("f32", 128)])
+def test_batch_equals_single_domain_over_graph_replays(storage, tile):
     doms = small_domains()
     rhos, us = random_init(doms)
-    print(f"batch vs single {storage}", flush=True)
+    print(f"batch vs single {storage} tile {tile}", flush=True)
     b = capi.Batch(doms, storage=storage)
+    b.set_tiling(tile, 0)
     b.init(np.concatenate(rhos), np.concatenate(us))
     singles = []
     for d, r, u in zip(doms, rhos, us):

@@ -341,3 +341,16 @@ def test_set_storage_conversions_exact():
     assert rc == 0 and np.isfinite(s.res_rel)
     fresh.close()
     lat.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+@pytest.mark.parametrize("nx", [65, 129])
+def test_neem_outlet_at_wave_start(storage, tol, nx):
+    """The +x NEEM face cell is the first lane of its wave (nx - 1 a multiple
+    of 64), so its inward neighbour is in the previous wave and the cell
+    gathers it from memory instead of taking it from the adjacent lane."""
+    shape = (nx, 12)
+    faces = {"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0), "-x": ("velocity_neem", (0.03, 0.0), 1.0),
+             "+x": ("pressure_neem", None, 0.999)}
+    eu, er, _ = run_pair(shape, 200, storage, 1 / 0.6, faces=faces, seed=8)
+    assert eu < tol and er < tol
