@@ -53,6 +53,12 @@ extern "C" {
                                     periodic) */
 #define LBM_FACE_VELOCITY_NEEM 6 /* non-equilibrium extrapolation velocity face, density from the inward
                                     neighbour (no reference counterpart; same neighbour rule) */
+#define LBM_FACE_PERIODIC_PRESSURE 7 /* 'periodic' with 'pressure' (src/lbm/bnd/bnd_periodic.h:102-119): a
+                                    population entering across the face is feq(p', u_c) + f_post(c) -
+                                    feq(rho_c, u_c) of the cell c straight across on the opposite face,
+                                    p' = rho of the opposite face; both faces of the axis carry it, and
+                                    it takes the lowest priority (walls win at corners, as the
+                                    reference's wall objects overwrite after its preApply) */
 
 /* cell flags (optional byte per cell) */
 #define LBM_CELL_FLUID 0

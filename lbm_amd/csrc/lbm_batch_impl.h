@@ -225,7 +225,7 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
     }
     for (int i = 0; i < dd.nfaces; ++i) {
       const lbm_bc_desc &f = dd.faces[i];
-      if (f.face < 0 || f.face >= 2 * b->dim || f.kind < 0 || f.kind > LBM_FACE_VELOCITY_NEEM) {
+      if (f.face < 0 || f.face >= 2 * b->dim || f.kind < 0 || f.kind > LBM_FACE_PERIODIC_PRESSURE) {
         bfail(nullptr, LBM_EINVAL, "lbm_batch_create: bad face in domain " + std::to_string(d));
         return bail(LBM_EINVAL);
       }
@@ -236,7 +236,8 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
     }
     for (int ax = 0; ax < 3; ++ax) {
       const bool p0 = D.face[2 * ax].kind == LBM_FACE_PERIODIC, p1 = D.face[2 * ax + 1].kind == LBM_FACE_PERIODIC;
-      if (ax < b->dim && p0 != p1) {
+      const bool q0 = D.face[2 * ax].kind == LBM_FACE_PERIODIC_PRESSURE, q1 = D.face[2 * ax + 1].kind == LBM_FACE_PERIODIC_PRESSURE;
+      if (ax < b->dim && (p0 != p1 || q0 != q1)) {
         bfail(nullptr, LBM_EINVAL, "lbm_batch_create: periodic face without partner");
         return bail(LBM_EINVAL);
       }

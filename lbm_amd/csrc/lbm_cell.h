@@ -225,6 +225,29 @@ __device__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C
   for (int j = 0; j < Q; ++j) h[j] = hn[j];
 }
 
+// density and velocity u* of the stored post-collision state at a cell,
+// accumulated load by load (no population array held in registers)
+template <int Q, typename ST>
+__device__ void cell_rho_u(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
+                           typename Store<ST>::C &rho, typename Store<ST>::C (&u)[3]) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  C d = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const C v = C(Store<ST>::ld(src + j * plane + cell));
+    d += v;
+    if (L::cx[j]) jx += C(L::cx[j]) * v;
+    if (L::cy[j]) jy += C(L::cy[j]) * v;
+    if (L::cz[j]) jz += C(L::cz[j]) * v;
+  }
+  rho = C(1) + d;
+  const C inv = C(1) / rho;
+  u[0] = (jx - C(0.5) * C(Fx)) * inv;
+  u[1] = (jy - C(0.5) * C(Fy)) * inv;
+  u[2] = (jz - C(0.5) * C(Fz)) * inv;
+}
+
 // velocity u* of the stored post-collision state at a cell (used for the
 // anti-bounce-back extrapolation: collision conserves rho and rho*u, so the
 // moments of S equal those of the pre-collision populations plus F).
@@ -394,6 +417,9 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
       // boundary substitution for populations whose upstream cell is missing
       C uext[3] = {0, 0, 0};
       int abb_face = -1;
+      int pp_face = -1;
+      long long pp_cell = 0;
+      C pp_rho = 1, pp_u[3] = {0, 0, 0};
 #pragma unroll
       for (int j = 0; j < Q - 1; ++j) {
         if (!((missing >> j) & 1u)) continue;
@@ -430,6 +456,23 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
           const C vsq = uext[0] * uext[0] + uext[1] * uext[1] + uext[2] * uext[2];
           // f = -f_post + 2 symmEq(p, u) in shifted form
           h[j] = R(-C(h[j]) + wj * (C(2) * C(fd.rho) * (C(1) + C(4.5) * cu * cu - C(1.5) * vsq) - C(2)));
+        }
+        else if (fd.kind == LBM_FACE_PERIODIC_PRESSURE) {
+          // the cell straight across on the opposite face supplies
+          // feq(p', u_c) + S(c, j) - feq(rho_c, u_c) (bnd_periodic.h:102-108)
+          if (pp_face != f) {
+            const int ax = f >> 1;
+            const int ox = ax == 0 ? ((f & 1) ? 0 : a.nx - 1) : x;
+            const int oy = ax == 1 ? ((f & 1) ? 0 : a.ny - 1) : y;
+            const int oz = ax == 2 ? ((f & 1) ? 0 : a.nz - 1) : z;
+            pp_cell = (long long)ox + (long long)a.px * ((long long)oy + (long long)a.ny * oz);
+            cell_rho_u<Q, ST>(src, plane, pp_cell, a.F[0], a.F[1], a.F[2], pp_rho, pp_u);
+            pp_face = f;
+          }
+          const C cu = C(cx) * pp_u[0] + C(cy) * pp_u[1] + C(cz) * pp_u[2];
+          const C usq = pp_u[0] * pp_u[0] + pp_u[1] * pp_u[1] + pp_u[2] * pp_u[2];
+          const C sj = C(Store<ST>::ld(src + (long long)j * plane + pp_cell));
+          h[j] = R(sj + wj * (C(a.face[f ^ 1].rho) - pp_rho) * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
         }
         // Zou-He faces: overwritten below
       }

@@ -467,7 +467,7 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const lbm_bc_desc &d = bcs[i];
     if (d.face < 0 || d.face >= 2 * h->dim) return fail(h, LBM_EINVAL, "lbm_set_bc: face index out of range");
     if (seen[d.face]) return fail(h, LBM_EINVAL, "lbm_set_bc: face described twice");
-    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_VELOCITY_NEEM) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
+    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_PERIODIC_PRESSURE) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
     seen[d.face] = true;
     face[d.face].kind = d.kind;
     face[d.face].prio = d.priority;
@@ -478,6 +478,8 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const bool p0 = face[2 * ax].kind == LBM_FACE_PERIODIC, p1 = face[2 * ax + 1].kind == LBM_FACE_PERIODIC;
     if (p0 != p1) return fail(h, LBM_EINVAL, "lbm_set_bc: a periodic face needs a periodic partner");
     per[ax] = p0 ? 1 : 0;
+    if ((face[2 * ax].kind == LBM_FACE_PERIODIC_PRESSURE) != (face[2 * ax + 1].kind == LBM_FACE_PERIODIC_PRESSURE))
+      return fail(h, LBM_EINVAL, "lbm_set_bc: a periodic pressure face needs a periodic pressure partner");
     const int na = ax == 0 ? h->nx : (ax == 1 ? h->ny : h->nz);
     for (int s = 0; s < 2; ++s) {
       const int k = face[2 * ax + s].kind;

@@ -128,8 +128,10 @@ auto LBMBndManager::lower(const CartesianGrid& grid) const -> DevicePlan {
     d.kind = r0.kind;
     for (int k = 0; k < 3; ++k) d.value[k] = r0.value[k];
     d.rho = r0.rho;
-    // later boundary objects overwrite earlier ones in the reference
-    d.priority = r0.order;
+    // later boundary objects overwrite earlier ones in the reference; a
+    // periodic pressure face writes before propagation (preApply), so every
+    // other object overwrites it
+    d.priority = r0.kind == LBM_FACE_PERIODIC_PRESSURE ? 0 : r0.order;
     if (r0.kind != LBM_FACE_PERIODIC) plan.faces.push_back(d);
   }
   for (int a = 0; a < dim; ++a) {

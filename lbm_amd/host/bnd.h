@@ -59,23 +59,35 @@ class LBMBnd_dummy : public LBMBndInterface {
 };
 
 // 'periodic' handled as a boundary object (generateBndry = true): copies the
-// outgoing populations to the linked cell, i.e. a wrap (bnd_periodic.h:102-119).
+// outgoing populations to the linked cell, i.e. a wrap (bnd_periodic.h:102-119);
+// with 'pressure' the copied set is re-based on that density
+// (feq(p, u) + f - feq), a periodic pair with a density jump.
 class LBMBnd_Periodic : public LBMBndInterface {
  public:
   LBMBnd_Periodic(const Surface* s, const json& props) : m_s(s) {
-    if (props.count("pressure")) TERMM(-1, "periodic boundary with a pressure jump is not supported by this build");
+    if (props.count("pressure")) {
+      m_pressure = required_value<GDouble>(props, "pressure");
+      m_hasPressure = true;
+    }
   }
   void initCnd(const VarAccess&) override {}
   [[nodiscard]] auto lower(GInt) const -> BndRule override {
     BndRule r;
-    r.kind = LBM_FACE_PERIODIC;
-    r.periodicViaBc = true;
+    if (m_hasPressure) {
+      r.kind = LBM_FACE_PERIODIC_PRESSURE;
+      r.rho = m_pressure;
+    } else {
+      r.kind = LBM_FACE_PERIODIC;
+      r.periodicViaBc = true;
+    }
     return r;
   }
   [[nodiscard]] auto surface() const -> const Surface* override { return m_s; }
 
  private:
   const Surface* m_s;
+  GDouble m_pressure = 1.0;
+  bool m_hasPressure = false;
 };
 
 // 'dirichlet'/'bounceback' (bnd_dirichlet.h:20-127): halfway bounce-back

@@ -155,6 +155,10 @@ int orc_macros(int Q, int64_t ncell, const double *fold, const double *force, do
  *   5 NEEM non-equilibrium extrapolation pressure face (new): all
  *          populations from the inward neighbour n1, density rho
  *   6 NEEM non-equilibrium extrapolation velocity face (new), velocity u
+ *   7 PP   periodic face with a density jump, LBMBndCell_periodic::preApply
+ *          (bnd_periodic.h:102-108): fold(c, o) = feq_o(rho, u_n1) +
+ *          fpost(n1, o) - feq_o(vars(n1)) for the missing directions, n1 the
+ *          linked cell on the other face, rho the pressure of n1's surface
  * dirmask bit i: apply to lattice direction i (the direction of the missing
  * neighbour), only consulted by BB and ABB. */
 typedef struct {
@@ -293,6 +297,19 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
       }
     } else if (r->kind == 3 || r->kind == 4) {
       zou_he_cell(&L, fold + c * Q, r->axis, r->sign, r->kind == 3, r->u, r->rho);
+    } else if (r->kind == 7) {
+      const double *v1 = vars + r->n1 * nv;
+      double vsq = 0;
+      for (int d = 0; d < L.dim; ++d) vsq += v1[d] * v1[d];
+      for (int i = 0; i < Q - 1; ++i) {
+        if (!((r->dirmask >> i) & 1)) continue;
+        if (nbr[c * (Q - 1) + i] >= 0) continue;
+        const int o = L.opp[i];
+        double cu = 0;
+        for (int d = 0; d < L.dim; ++d) cu += L.c[o][d] * v1[d];
+        fold[c * Q + o] = default_eq(L.w[o], r->rho, cu, vsq) + fpost[r->n1 * Q + o] -
+                          default_eq(L.w[o], v1[L.dim], cu, vsq);
+      }
     } else if (r->kind == 5 || r->kind == 6) {
       /* NEEM faces (Guo, Zheng & Shi, Chin. Phys. 11 366 (2002)): every
        * population of c from the inward neighbour n1 (complete after the

@@ -23,7 +23,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblbm_oracle.so")
 REF_HILBERT = os.path.join(HERE, "_ref", "libref_hilbert.so")
 
-FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6}
+FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6,
+        "periodic_pressure": 7}
 
 
 class OrcBC(ct.Structure):
@@ -201,6 +202,13 @@ class Problem:
                     r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
                     r.n2 = c2[0] + nx * (c2[1] + ny * c2[2])
                     recs.append(r)
+                elif kind == FACE["periodic_pressure"]:
+                    r.kind = 7
+                    r.rho = self.face_rho[key ^ 1]  # pressure of the other face
+                    c1 = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+                    c1[ax] = 0 if side == 1 else ext[ax] - 1
+                    r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
+                    recs.append(r)
                 elif kind in (FACE["pressure_neem"], FACE["velocity_neem"]):
                     # the face record (below) replaces every population; with
                     # a solid inward neighbour the missing ones bounce back
@@ -354,7 +362,7 @@ class GridProblem(Problem):
     no-slip walls (interior obstacles); Zou-He records always come last.
     """
 
-    def __init__(self, dim, nbr_table, surfaces, bcs, omega, force=(0.0, 0.0, 0.0), textbook=False):
+    def __init__(self, dim, nbr_table, surfaces, bcs, omega, force=(0.0, 0.0, 0.0), textbook=False, coords=None):
         self.dim = dim
         self.Q = 9 if dim == 2 else 19
         n = nbr_table.shape[0]
@@ -369,11 +377,38 @@ class GridProblem(Problem):
         dv3 = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
         dv = dv2 if dim == 2 else dv3
         recs, zh = [], []
+        pre = []  # periodic pressure records: the reference's preApply runs before every apply
         gated = np.zeros((n, self.Q - 1), dtype=bool)
         self.init_u = {}
         for name, spec in bcs:
             cells, dirs = surfaces[name]
             t = spec["type"]
+            if t == "periodic" and "pressure" in spec and spec.get("generateBndry", True):
+                # LBMBndCell_periodic::preApply (bnd_periodic.h:102-108): every
+                # cell c of this surface writes the cell L straight across
+                # on the connected surface, with this surface's pressure
+                assert coords is not None, "periodic pressure needs the cells' structured coordinates"
+                co = np.asarray(coords)
+                ext = co.max(axis=0) + 1
+                key = {tuple(v): k for k, v in enumerate(co.tolist())}
+                for cidx, d in zip(cells, dirs):
+                    a = d // 2
+                    tgt = list(co[cidx])
+                    tgt[a] = ext[a] - 1 if d % 2 == 0 else 0
+                    L = key[tuple(tgt)]
+                    r = OrcBC()
+                    r.cell = int(L)
+                    r.kind = 7
+                    r.n1 = int(cidx)
+                    r.rho = float(spec["pressure"])
+                    mask = 0
+                    for i in range(self.Q - 1):
+                        if self.nbr[L, i] < 0:
+                            mask |= 1 << i
+                            gated[L, i] = True
+                    r.dirmask = mask
+                    pre.append(r)
+                continue
             if t == "periodic" or not spec.get("generateBndry", True):
                 continue
             for cidx, d in zip(cells, dirs):
@@ -444,7 +479,7 @@ class GridProblem(Problem):
                     r.dirmask = mask
                     extra.append(r)
             recs = extra + recs
-        allr = recs + zh
+        allr = pre + recs + zh
         self.nrec = len(allr)
         self.recs = (OrcBC * max(1, len(allr)))(*allr)
 

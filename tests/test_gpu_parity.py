@@ -354,3 +354,14 @@ def test_neem_outlet_at_wave_start(storage, tol, nx):
              "+x": ("pressure_neem", None, 0.999)}
     eu, er, _ = run_pair(shape, 200, storage, 1 / 0.6, faces=faces, seed=8)
     assert eu < tol and er < tol
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+@pytest.mark.parametrize("shape", [(40, 14), (24, 10, 8)])
+def test_periodic_pressure_channel(storage, tol, shape):
+    """Periodic pair with a density jump driving a walled channel."""
+    faces = {"-x": ("periodic_pressure", None, 1.002), "+x": ("periodic_pressure", None, 1.0),
+             "-y": ("wall", None, 1.0), "+y": ("wall", (0.01, 0.0, 0.0), 1.0)}
+    eu, er, _ = run_pair(shape, 150, storage, 1 / 0.7, faces=faces, seed=12)
+    print(f"periodic pressure {shape} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol

@@ -419,3 +419,48 @@ def test_neem_outlet_converges_behind_cylinder_at_low_viscosity():
     # the slow density mode still drifts at ~1e-6 of the flux per section
     flux = (rho * ub[:, 0]).reshape(ny, nx).sum(axis=0)
     assert np.abs(flux / flux[0] - 1).max() < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(7, 5), (6, 4, 5)])
+def test_periodic_pressure_face_brute_force(shape):
+    """Periodic pair with a density jump (bnd_periodic.h:102-108), checked with
+    omega = 0: a population entering a face cell L across the face equals
+    feq(p', u_c) + f(c) - feq(rho_c, u_c) of the cell c straight across on the
+    other face, p' the pressure of c's face; walls win at the corners."""
+    dim = len(shape)
+    p_lo, p_hi = 1.004, 0.997
+    faces = {"-x": ("periodic_pressure", None, p_lo), "+x": ("periodic_pressure", None, p_hi),
+             "-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0)}
+    p = O.Problem(shape, faces=faces, omega=0.0)
+    full = tuple(shape) + ((1,) if dim == 2 else ())
+    nx, ny, nz = full
+    N = nx * ny * nz
+    rng = np.random.default_rng(6)
+    p.init(1 + 0.05 * rng.standard_normal(N), 0.05 * rng.standard_normal((N, dim)))
+    Q = p.Q
+    f0 = p.fold.reshape(N, Q).copy()
+    p.step(1)
+    f1 = p.fold.reshape(N, Q)
+    c, w, opp = O.tables(Q)
+
+    def feq(rho, u):
+        cu = c[:, :dim] @ u
+        return w * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * (u @ u))
+
+    checked = 0
+    for z in range(nz):
+        for y in range(ny):
+            for x, xo, pp in ((nx - 1, 0, p_lo), (0, nx - 1, p_hi)):
+                cell = x + nx * (y + ny * z)
+                other = xo + nx * (y + ny * z)
+                rho_c = f0[other].sum()
+                u_c = (c[:, :dim] * f0[other][:, None]).sum(axis=0) / rho_c
+                ref = feq(pp, u_c) + f0[other] - feq(rho_c, u_c)
+                for o in range(Q - 1):
+                    if int(c[o, 0]) != (-1 if x == nx - 1 else 1):
+                        continue  # not entering across this x face
+                    if not (0 <= y - int(c[o, 1]) < ny):
+                        continue  # crosses a wall too: the wall wins
+                    assert abs(f1[cell, o] - ref[o]) < 1e-15, (x, y, z, o)
+                    checked += 1
+    assert checked > 0
