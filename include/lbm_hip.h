@@ -59,6 +59,18 @@ extern "C" {
                                     p' = rho of the opposite face; both faces of the axis carry it, and
                                     it takes the lowest priority (walls win at corners, as the
                                     reference's wall objects overwrite after its preApply) */
+/* Wet-node walls: every population of a face cell is rebuilt after streaming
+ * from the density of the populations known there (LBMBnd_wallWetnode,
+ * src/lbm/bnd/bnd_wetnode.h:20-55; calcDensity_limited,
+ * src/lbm/moments.h:120-153) and the wall velocity `value`; populations whose
+ * upstream face has a higher priority keep that face's rule. */
+#define LBM_FACE_WALL_EQ 8   /* 'wall'/'equilibrium': feq(rho, u_w) (src/lbm/bnd/bnd_wall.h:100-185) */
+#define LBM_FACE_WALL_NEEM 9 /* 'wall'/'neem': feq(rho, u_w) + f(n1) - feq(rho1, u1), n1 the inward
+                                neighbour (src/lbm/bnd/bnd_wall.h:187-300); at least 3 cells along the
+                                normal, no wet-node, Zou-He or NEEM face next to it */
+#define LBM_FACE_WALL_NEBB 10 /* 'wall'/'nebb', D2Q9 only: normal population reflected, diagonals
+                                 from the known ones (src/lbm/bnd/bnd_wall.h:303-479); the wall
+                                 velocity must be tangential */
 
 /* cell flags (optional byte per cell) */
 #define LBM_CELL_FLUID 0

@@ -38,7 +38,10 @@ struct KArgs {
   long long plane;       // elements between population planes
   int nx, ny, nz, px;
   int per[3];
-  FaceDev face[6];
+  // the six faces, in device memory: indexed at run time in the boundary
+  // paths (a table among the kernel arguments would then be copied to
+  // scratch memory by every thread)
+  const FaceDev *face;
   double omega;
   double F[3];
   void *snap_in;
@@ -173,7 +176,7 @@ __device__ __forceinline__ void collide_store(const typename Store<ST>::R (&h)[Q
 // outward sign of the face normal; velocity face prescribes u, pressure
 // face prescribes rho with zero tangential velocity.
 template <int Q, typename C>
-__device__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C u2, C rho_p) {
+__device__ __forceinline__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C u2, C rho_p) {
   using L = Lat<Q>;
   auto cax = [](int j, int a) { return a == 0 ? L::cx[j] : (a == 1 ? L::cy[j] : L::cz[j]); };
   C s0 = 0, sout = 0;
@@ -228,7 +231,7 @@ __device__ void zou_he(C (&h)[Q], int axis, int sign, bool is_vel, C u0, C u1, C
 // density and velocity u* of the stored post-collision state at a cell,
 // accumulated load by load (no population array held in registers)
 template <int Q, typename ST>
-__device__ void cell_rho_u(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
+__device__ __forceinline__ void cell_rho_u(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
                            typename Store<ST>::C &rho, typename Store<ST>::C (&u)[3]) {
   using L = Lat<Q>;
   using C = typename Store<ST>::C;
@@ -252,7 +255,7 @@ __device__ void cell_rho_u(const ST *src, long long plane, long long cell, doubl
 // anti-bounce-back extrapolation: collision conserves rho and rho*u, so the
 // moments of S equal those of the pre-collision populations plus F).
 template <int Q, typename ST>
-__device__ void cell_velocity(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
+__device__ __forceinline__ void cell_velocity(const ST *src, long long plane, long long cell, double Fx, double Fy, double Fz,
                               typename Store<ST>::C (&u)[3]) {
   using C = typename Store<ST>::C;
   typename Store<ST>::R h[Q];
@@ -276,7 +279,7 @@ template <int Q, typename ST, typename G>
 __device__ __forceinline__ void neem_correct(const G &a, int f, typename Store<ST>::R (&h)[Q]);
 
 template <int Q, typename ST, bool FLAGS, typename G>
-__device__ void neem_face(const G &a, int x, int y, int z, int f, typename Store<ST>::R (&h)[Q]) {
+__device__ __forceinline__ void neem_face(const G &a, int x, int y, int z, int f, typename Store<ST>::R (&h)[Q]) {
   using L = Lat<Q>;
   using C = typename Store<ST>::C;
   using R = typename Store<ST>::R;
@@ -347,6 +350,211 @@ __device__ __forceinline__ void neem_correct(const G &a, int f, typename Store<S
   }
 }
 
+__device__ __forceinline__ bool full_cell_kind(int k) {
+  return k == LBM_FACE_VELOCITY_ZH || k == LBM_FACE_PRESSURE_ZH || k == LBM_FACE_PRESSURE_NEEM ||
+         k == LBM_FACE_VELOCITY_NEEM || k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB;
+}
+__device__ __forceinline__ bool wet_kind(int k) {
+  return k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB;
+}
+
+// Post-streaming population j of the cell (x1, y1, z1) (index n1, link mask
+// lm1): pulled from its upstream cell; across a wall face bounced back with
+// the Ladd term; across the periodic-pressure face pg the value that face
+// supplies, feq(p', u_c) + S(c, j) - feq(rho_c, u_c) with the moments
+// (rc, uc) of the cell pc straight across; from a solid or across any other
+// face plain bounce-back S(n1, opp j).
+template <int Q, typename ST, bool FLAGS, typename G>
+__device__ __forceinline__ typename Store<ST>::C post_stream_pop(const G &a, int x1, int y1, int z1, long long n1,
+                                                                 uint32_t lm1, int j, int pg, long long pc,
+                                                                 typename Store<ST>::C rc,
+                                                                 const typename Store<ST>::C (&uc)[3]) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  const ST *src = static_cast<const ST *>(a.src);
+  const long long plane = a.plane;
+  const int ext[3] = {a.nx, a.ny, a.nz};
+  const int cj[3] = {L::cx[j], L::cy[j], L::cz[j]};
+  int s[3] = {x1 - cj[0], y1 - cj[1], z1 - cj[2]};
+  int face = -1;
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) {
+    if (cj[d] == 0) continue;
+    if (s[d] >= 0 && s[d] < ext[d]) continue;
+    if (a.per[d]) { s[d] = s[d] < 0 ? s[d] + ext[d] : s[d] - ext[d]; continue; }
+    ok = false;
+    const int g = 2 * d + (cj[d] > 0 ? 0 : 1);
+    if (face < 0 || a.face[g].prio > a.face[face].prio) face = g;
+  }
+  if (ok && FLAGS && j != Q - 1 && ((lm1 >> j) & 1u)) ok = false;
+  const C wj = C(L::w[j]);
+  long long addr = ok ? (long long)j * plane + (long long)s[0] + (long long)a.px * ((long long)s[1] + (long long)a.ny * s[2])
+                      : (long long)L::opp[j] * plane + n1;
+  if (!ok && face >= 0 && face == pg) addr = (long long)j * plane + pc;
+  C v = C(Store<ST>::ld(src + addr));
+  if (ok) return v;
+  if (face >= 0 && face == pg) {
+    const C cu = C(cj[0]) * uc[0] + C(cj[1]) * uc[1] + C(cj[2]) * uc[2];
+    const C usq = uc[0] * uc[0] + uc[1] * uc[1] + uc[2] * uc[2];
+    return v + wj * (C(a.face[face ^ 1].rho) - rc) * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
+  }
+  if (face >= 0 && a.face[face].kind == LBM_FACE_WALL)
+    v += C(6) * wj * (C(cj[0]) * C(a.face[face].u[0]) + C(cj[1]) * C(a.face[face].u[1]) + C(cj[2]) * C(a.face[face].u[2]));
+  return v;
+}
+
+// Wet-node walls (src/lbm/bnd/bnd_wall.h:100-479) on the cell's face f.
+// Density from the populations known at the cell (LBMBnd_wallWetnode,
+// bnd_wetnode.h:20-55, and calcDensity_limited, moments.h:120-153): a
+// population leaving through the wall counts twice, one parallel to it once
+// (twice when the cell's downstream neighbour along it is missing), each only
+// when it streamed in or entered across a periodic-pressure face (ppm);
+// when the constants do not add up to Q-1 (a corner) the density is the
+// cell's density before the step.  With a wall velocity u_w the density is
+// divided by 1 + n.u_w (n the outward normal).  Then
+//   equilibrium: h_i = heq_i(rho, u_w);
+//   NEEM:        h_i = heq_i(rho, u_w) + h_i(n1) - heq_i(rho1, u1), n1 the
+//                inward neighbour with its post-streaming populations and
+//                their moments (no force term, as calcVelocity);
+//   NEBB (D2Q9): the inward normal population = the outward one + s 2/3 rho
+//                u_w,n, the two inward diagonals from the outward ones, the
+//                tangential difference and u_w (Zou & He 1997), the other
+//                populations unchanged.
+// Populations in `hold` (upstream face of higher priority) keep their value.
+template <int Q, typename ST, bool FLAGS, bool W64, typename G>
+__device__ __forceinline__ void wet_wall(const G &a, int x, int y, int z, int f, typename Store<ST>::R (&h)[Q], unsigned missing,
+                         unsigned ppm, unsigned hold, Idx<W64> cell) {
+  using L = Lat<Q>;
+  using C = typename Store<ST>::C;
+  using R = typename Store<ST>::R;
+  const ST *src = static_cast<const ST *>(a.src);
+  const long long plane = a.plane;
+  const int ax = f >> 1, s = (f & 1) ? 1 : -1;
+  const int k = a.face[f].kind;
+  auto cax = [](int j, int d) { return d == 0 ? L::cx[j] : (d == 1 ? L::cy[j] : L::cz[j]); };
+  C dr = 0, ws = 0;
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < Q - 1; ++j) {
+    const int cn = cax(j, ax) * s;
+    if (cn < 0) continue;
+    if (((missing >> j) & 1u) && !((ppm >> j) & 1u)) continue;
+    int kk = 2;
+    if (cn == 0) {
+      const int o = L::opp[j];
+      kk = (((missing >> o) & 1u) && !((ppm >> o) & 1u)) ? 2 : 1;
+    }
+    dr += C(kk) * C(h[j]);
+    ws += C(kk) * C(L::w[j]);
+    cnt += kk;
+  }
+  C drho;
+  if (cnt == Q - 1) {
+    drho = dr + C(h[Q - 1]) + (ws + C(L::w[Q - 1]) - C(1));
+  } else {
+    drho = 0;
+#pragma unroll
+    for (int j = 0; j < Q; ++j) drho += C(Store<ST>::ld(pop_ptr<ST, W64>(src, plane, j, cell)));
+  }
+  const C uw[3] = {C(a.face[f].u[0]), C(a.face[f].u[1]), C(a.face[f].u[2])};
+  const C uwn = ax == 0 ? uw[0] : (ax == 1 ? uw[1] : uw[2]);  // no run-time array index
+  C rho = C(1) + drho;
+  const C un = C(s) * uwn;
+  if (un != C(0)) {
+    rho = rho * (C(1) / (C(1) + un));
+    drho = rho - C(1);
+  }
+  const C usq = uw[0] * uw[0] + uw[1] * uw[1] + uw[2] * uw[2];
+  if (k == LBM_FACE_WALL_NEBB) {
+    if constexpr (Q == 9) {
+      const int out = 2 * ax + (s > 0 ? 1 : 0), in = L::opp[out];
+      if (!((hold >> in) & 1u)) h[in] = R(C(h[out]) + C(s) * C(2.0 / 3.0) * rho * uwn);
+      const C sixth = C(1.0 / 6.0);
+      C n[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) n[j] = C(h[j]);
+      if (ax == 0 && s > 0) {
+        n[6] = C(h[4]) + C(0.5) * (C(h[3]) - C(h[2])) - C(0.5) * rho * uw[1] - sixth * rho * uw[0];
+        n[7] = C(h[5]) - C(0.5) * (C(h[3]) - C(h[2])) + C(0.5) * rho * uw[1] - sixth * rho * uw[0];
+      } else if (ax == 0) {
+        n[4] = C(h[6]) - C(0.5) * (C(h[3]) - C(h[2])) + C(0.5) * rho * uw[1] + sixth * rho * uw[0];
+        n[5] = C(h[7]) + C(0.5) * (C(h[3]) - C(h[2])) - C(0.5) * rho * uw[1] + sixth * rho * uw[0];
+      } else if (s > 0) {
+        n[5] = C(h[7]) - C(0.5) * (C(h[1]) - C(h[0])) + C(0.5) * rho * uw[0] - sixth * rho * uw[1];
+        n[6] = C(h[4]) + C(0.5) * (C(h[1]) - C(h[0])) - C(0.5) * rho * uw[0] - sixth * rho * uw[1];
+      } else {
+        n[7] = C(h[5]) + C(0.5) * (C(h[1]) - C(h[0])) - C(0.5) * rho * uw[0] + sixth * rho * uw[1];
+        n[4] = C(h[6]) - C(0.5) * (C(h[1]) - C(h[0])) + C(0.5) * rho * uw[0] + sixth * rho * uw[1];
+      }
+#pragma unroll
+      for (int j = 4; j < 8; ++j)
+        if (!((hold >> j) & 1u)) h[j] = R(n[j]);
+    }
+    return;
+  }
+  // NEEM: moments of the inward neighbour's post-streaming populations
+  bool neem = k == LBM_FACE_WALL_NEEM;
+  int x1 = x, y1 = y, z1 = z;
+  if (ax == 0) x1 -= s;
+  if (ax == 1) y1 -= s;
+  if (ax == 2) z1 -= s;
+  const long long n1 = (long long)x1 + (long long)a.px * ((long long)y1 + (long long)a.ny * z1);
+  const uint32_t lm1 = (FLAGS && neem) ? a.links[n1] : 0u;
+  if (lm1 & LINK_SELF_SOLID) neem = false;
+  // the periodic-pressure face n1 lies on, if any, and the moments of the
+  // cell straight across from n1 on it
+  int pg = -1;
+  long long pc = 0;
+  C rc = 1, uc[3] = {0, 0, 0};
+  if (neem) {
+    const int c1[3] = {x1, y1, z1}, ext[3] = {a.nx, a.ny, a.nz};
+    int cx = x1, cy = y1, cz = z1;
+#pragma unroll
+    for (int g = 0; g < 2 * L::D; ++g) {
+      const int ag = g >> 1;
+      if (pg >= 0 || ag == ax || a.face[g].kind != LBM_FACE_PERIODIC_PRESSURE) continue;
+      if (c1[ag] != ((g & 1) ? ext[ag] - 1 : 0)) continue;
+      const int o = (g & 1) ? 0 : ext[ag] - 1;
+      if (ag == 0) cx = o;
+      if (ag == 1) cy = o;
+      if (ag == 2) cz = o;
+      pg = g;
+    }
+    if (pg >= 0) {
+      pc = (long long)cx + (long long)a.px * ((long long)cy + (long long)a.ny * cz);
+      cell_rho_u<Q, ST>(src, plane, pc, a.F[0], a.F[1], a.F[2], rc, uc);
+    }
+  }
+  C d1 = 0, j1[3] = {0, 0, 0};
+  if (neem) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const C v = post_stream_pop<Q, ST, FLAGS>(a, x1, y1, z1, n1, lm1, j, pg, pc, rc, uc);
+      d1 += v;
+      j1[0] += C(L::cx[j]) * v;
+      j1[1] += C(L::cy[j]) * v;
+      j1[2] += C(L::cz[j]) * v;
+    }
+  }
+  const C rho1 = C(1) + d1;
+  const C u1[3] = {j1[0] / rho1, j1[1] / rho1, j1[2] / rho1};
+  const C usq1 = u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    if ((hold >> j) & 1u) continue;
+    const C wj = C(L::w[j]);
+    const C cu = C(L::cx[j]) * uw[0] + C(L::cy[j]) * uw[1] + C(L::cz[j]) * uw[2];
+    C v = wj * (drho + rho * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+    if (neem) {
+      const C c1 = C(L::cx[j]) * u1[0] + C(L::cy[j]) * u1[1] + C(L::cz[j]) * u1[2];
+      v += post_stream_pop<Q, ST, FLAGS>(a, x1, y1, z1, n1, lm1, j, pg, pc, rc, uc) -
+           wj * (d1 + rho1 * (C(3) * c1 + C(4.5) * c1 * c1 - C(1.5) * usq1));
+    }
+    h[j] = R(v);
+  }
+}
+
 // --------------------------------------------------------------------------
 // The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
 // 2: reduce against the snapshot written by the previous step; 3: both
@@ -372,6 +580,12 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
   using R = typename Store<ST>::R;
   R h[Q];
   unsigned missing = 0;
+  // ff: the face whose rule replaces the whole cell (Zou-He, NEEM, wet-node
+  // walls), the one of highest priority the cell lies on, ties to the lower
+  // face index; hold: populations whose upstream face outranks a wet-node
+  // wall ff; ppm: populations entering across a periodic-pressure face
+  int ff = -1;
+  unsigned hold = 0, ppm = 0;
   if (active) {
     // neighbour coordinates per axis with periodic wrap; v* = exists
     int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
@@ -410,16 +624,30 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
     if (FLAGS) missing |= lm & ((1u << (Q - 1)) - 1u);
 
     if (missing) {
+      // bit f: the cell lies on face f
+      const unsigned on = (vxm ? 0u : 1u) | (vxp ? 0u : 2u) | (vym ? 0u : 4u) | (vyp ? 0u : 8u) | (vzm ? 0u : 16u) |
+                          (vzp ? 0u : 32u);
+      // (kind and priority of ff tracked here: the face table is indexed
+      // with compile-time indices only)
+      int ffprio = 0, ffkind = -1;
+#pragma unroll
+      for (int f = 0; f < 2 * L::D; ++f)
+        if (((on >> f) & 1u) && full_cell_kind(a.face[f].kind) && (ff < 0 || a.face[f].prio > ffprio)) {
+          ff = f;
+          ffprio = a.face[f].prio;
+          ffkind = a.face[f].kind;
+        }
+      const bool wet = wet_kind(ffkind);
       // bounce-back source S(x, opp j) of every missing population
 #pragma unroll
       for (int j = 0; j < Q - 1; ++j)
         if ((missing >> j) & 1u) h[j] = Store<ST>::ld(pop_ptr<ST, W64>(src, plane, L::opp[j], cell));
-      // boundary substitution for populations whose upstream cell is missing
-      C uext[3] = {0, 0, 0};
-      int abb_face = -1;
-      int pp_face = -1;
-      long long pp_cell = 0;
-      C pp_rho = 1, pp_u[3] = {0, 0, 0};
+      // boundary substitution for populations whose upstream cell is missing.
+      // First pass: the face each one crosses (the higher priority wins,
+      // ties to the lower axis), walls applied at once; anti-bounce-back
+      // and periodic-pressure faces read other cells and go in passes of
+      // their own, so that their state is live only there.
+      unsigned abbm = 0;
 #pragma unroll
       for (int j = 0; j < Q - 1; ++j) {
         if (!((missing >> j) & 1u)) continue;
@@ -428,53 +656,92 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
         const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
         const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
         int f = -1;
-        // crossed faces: the higher priority wins, ties to the lower axis
         if (ix) f = cx > 0 ? 0 : 1;
         if (iy) { const int g = cy > 0 ? 2 : 3; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
         if (iz) { const int g = cz > 0 ? 4 : 5; if (f < 0 || a.face[g].prio > a.face[f].prio) f = g; }
         if (f < 0) continue;  // solid obstacle: plain bounce-back, h[j] already holds S(x, opp j)
-        const FaceDev &fd = a.face[f];
-        const C wj = C(L::w[j]);
-        if (fd.kind == LBM_FACE_WALL) {
+        const int fk = a.face[f].kind;
+        if (wet && a.face[f].prio > ffprio) hold |= 1u << j;
+        if (fk == LBM_FACE_WALL) {
           // Ladd moving wall, rho0 = 1 (bnd_dirichlet.h:109-116)
-          const C cu = C(cx) * C(fd.u[0]) + C(cy) * C(fd.u[1]) + C(cz) * C(fd.u[2]);
-          h[j] = R(C(h[j]) + C(6) * wj * cu);
-        } else if (fd.kind == LBM_FACE_PRESSURE_ABB) {
-          if (abb_face != f) {
-            // u extrapolated from the two inward cells (bnd_pressure.h:70-81)
-            const int ax = f >> 1, out = (f & 1) ? 1 : -1;
-            const int dx = ax == 0 ? -out : 0, dy = ax == 1 ? -out : 0, dz = ax == 2 ? -out : 0;
-            const long long n1 = (long long)(x + dx) + (long long)a.px * ((long long)(y + dy) + (long long)a.ny * (z + dz));
-            const long long n2 = (long long)(x + 2 * dx) + (long long)a.px * ((long long)(y + 2 * dy) + (long long)a.ny * (z + 2 * dz));
-            C u1[3], u2[3];
-            cell_velocity<Q, ST>(src, plane, n1, a.F[0], a.F[1], a.F[2], u1);
-            cell_velocity<Q, ST>(src, plane, n2, a.F[0], a.F[1], a.F[2], u2);
-            for (int d = 0; d < 3; ++d) uext[d] = C(1.5) * u1[d] - C(0.5) * u2[d];
-            abb_face = f;
-          }
-          const C cu = C(cx) * uext[0] + C(cy) * uext[1] + C(cz) * uext[2];
+          const C cu = C(cx) * C(a.face[f].u[0]) + C(cy) * C(a.face[f].u[1]) + C(cz) * C(a.face[f].u[2]);
+          h[j] = R(C(h[j]) + C(6) * C(L::w[j]) * cu);
+        } else if (fk == LBM_FACE_PRESSURE_ABB) {
+          abbm |= 1u << j;
+        } else if (fk == LBM_FACE_PERIODIC_PRESSURE) {
+          ppm |= 1u << j;
+        }
+        // Zou-He, NEEM and wet-node faces: whole-cell rule below
+      }
+      // anti-bounce-back, per face the cell lies on: u extrapolated from the
+      // two inward cells (bnd_pressure.h:70-81), f = -f_post + 2 symmEq(p, u)
+      if (abbm) {
+#pragma unroll 1
+        for (int f = 0; f < 2 * L::D; ++f) {
+          if (!((on >> f) & 1u) || a.face[f].kind != LBM_FACE_PRESSURE_ABB) continue;
+          const int ax = f >> 1, out = (f & 1) ? 1 : -1;
+          const int dx = ax == 0 ? -out : 0, dy = ax == 1 ? -out : 0, dz = ax == 2 ? -out : 0;
+          const long long n1 = (long long)(x + dx) + (long long)a.px * ((long long)(y + dy) + (long long)a.ny * (z + dz));
+          const long long n2 = (long long)(x + 2 * dx) + (long long)a.px * ((long long)(y + 2 * dy) + (long long)a.ny * (z + 2 * dz));
+          C u1[3], u2[3], uext[3];
+          cell_velocity<Q, ST>(src, plane, n1, a.F[0], a.F[1], a.F[2], u1);
+          cell_velocity<Q, ST>(src, plane, n2, a.F[0], a.F[1], a.F[2], u2);
+          for (int d = 0; d < 3; ++d) uext[d] = C(1.5) * u1[d] - C(0.5) * u2[d];
           const C vsq = uext[0] * uext[0] + uext[1] * uext[1] + uext[2] * uext[2];
-          // f = -f_post + 2 symmEq(p, u) in shifted form
-          h[j] = R(-C(h[j]) + wj * (C(2) * C(fd.rho) * (C(1) + C(4.5) * cu * cu - C(1.5) * vsq) - C(2)));
-        }
-        else if (fd.kind == LBM_FACE_PERIODIC_PRESSURE) {
-          // the cell straight across on the opposite face supplies
-          // feq(p', u_c) + S(c, j) - feq(rho_c, u_c) (bnd_periodic.h:102-108)
-          if (pp_face != f) {
-            const int ax = f >> 1;
-            const int ox = ax == 0 ? ((f & 1) ? 0 : a.nx - 1) : x;
-            const int oy = ax == 1 ? ((f & 1) ? 0 : a.ny - 1) : y;
-            const int oz = ax == 2 ? ((f & 1) ? 0 : a.nz - 1) : z;
-            pp_cell = (long long)ox + (long long)a.px * ((long long)oy + (long long)a.ny * oz);
-            cell_rho_u<Q, ST>(src, plane, pp_cell, a.F[0], a.F[1], a.F[2], pp_rho, pp_u);
-            pp_face = f;
+          const C p2 = C(2) * C(a.face[f].rho);
+#pragma unroll
+          for (int j = 0; j < Q - 1; ++j) {
+            if (!((abbm >> j) & 1u)) continue;
+            if (!(ax == 0 ? L::cx[j] == -out : (ax == 1 ? L::cy[j] == -out : L::cz[j] == -out))) continue;
+            // owned by this face unless a face of higher priority won it
+            const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+            const bool ix = (cx > 0 && !vxm) || (cx < 0 && !vxp);
+            const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
+            const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
+            int g = -1;
+            if (ix) g = cx > 0 ? 0 : 1;
+            if (iy) { const int t = cy > 0 ? 2 : 3; if (g < 0 || a.face[t].prio > a.face[g].prio) g = t; }
+            if (iz) { const int t = cz > 0 ? 4 : 5; if (g < 0 || a.face[t].prio > a.face[g].prio) g = t; }
+            if (g != f) continue;
+            const C cu = C(cx) * uext[0] + C(cy) * uext[1] + C(cz) * uext[2];
+            h[j] = R(-C(h[j]) + C(L::w[j]) * (p2 * (C(1) + C(4.5) * cu * cu - C(1.5) * vsq) - C(2)));
           }
-          const C cu = C(cx) * pp_u[0] + C(cy) * pp_u[1] + C(cz) * pp_u[2];
-          const C usq = pp_u[0] * pp_u[0] + pp_u[1] * pp_u[1] + pp_u[2] * pp_u[2];
-          const C sj = C(Store<ST>::ld(src + (long long)j * plane + pp_cell));
-          h[j] = R(sj + wj * (C(a.face[f ^ 1].rho) - pp_rho) * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
         }
-        // Zou-He faces: overwritten below
+      }
+      // periodic pressure, per face: the cell straight across on the opposite
+      // face supplies feq(p', u_c) + S(c, j) - feq(rho_c, u_c)
+      // (bnd_periodic.h:102-108)
+      if (ppm) {
+#pragma unroll 1
+        for (int f = 0; f < 2 * L::D; ++f) {
+          if (!((on >> f) & 1u) || a.face[f].kind != LBM_FACE_PERIODIC_PRESSURE) continue;
+          const int ax = f >> 1, out = (f & 1) ? 1 : -1;
+          const int ox = ax == 0 ? ((f & 1) ? 0 : a.nx - 1) : x;
+          const int oy = ax == 1 ? ((f & 1) ? 0 : a.ny - 1) : y;
+          const int oz = ax == 2 ? ((f & 1) ? 0 : a.nz - 1) : z;
+          const long long pc = (long long)ox + (long long)a.px * ((long long)oy + (long long)a.ny * oz);
+          C prho, pu[3];
+          cell_rho_u<Q, ST>(src, plane, pc, a.F[0], a.F[1], a.F[2], prho, pu);
+          const C usq = pu[0] * pu[0] + pu[1] * pu[1] + pu[2] * pu[2];
+          const C dp = C(a.face[f ^ 1].rho) - prho;
+#pragma unroll
+          for (int j = 0; j < Q - 1; ++j) {
+            if (!((ppm >> j) & 1u)) continue;
+            if (!(ax == 0 ? L::cx[j] == -out : (ax == 1 ? L::cy[j] == -out : L::cz[j] == -out))) continue;
+            const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+            const bool ix = (cx > 0 && !vxm) || (cx < 0 && !vxp);
+            const bool iy = (cy > 0 && !vym) || (cy < 0 && !vyp);
+            const bool iz = (cz > 0 && !vzm) || (cz < 0 && !vzp);
+            int g = -1;
+            if (ix) g = cx > 0 ? 0 : 1;
+            if (iy) { const int t = cy > 0 ? 2 : 3; if (g < 0 || a.face[t].prio > a.face[g].prio) g = t; }
+            if (iz) { const int t = cz > 0 ? 4 : 5; if (g < 0 || a.face[t].prio > a.face[g].prio) g = t; }
+            if (g != f) continue;
+            const C cu = C(cx) * pu[0] + C(cy) * pu[1] + C(cz) * pu[2];
+            const C sj = C(Store<ST>::ld(src + (long long)j * plane + pc));
+            h[j] = R(sj + C(L::w[j]) * dp * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq));
+          }
+        }
       }
     }
   }
@@ -510,26 +777,17 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
   if (active) {
     const C Fx = C(a.F[0]), Fy = C(a.F[1]), Fz = C(a.F[2]);
     const bool force = a.F[0] != 0.0 || a.F[1] != 0.0 || a.F[2] != 0.0;
-    if (missing) {
-      // Zou-He or NEEM on the first such face the cell lies on
-#pragma unroll 1
-      for (int f = 0; f < 2 * L::D; ++f) {
-        const int k = a.face[f].kind;
-        if (k != LBM_FACE_VELOCITY_ZH && k != LBM_FACE_PRESSURE_ZH && k != LBM_FACE_PRESSURE_NEEM &&
-            k != LBM_FACE_VELOCITY_NEEM)
-          continue;
-        const int ax = f >> 1;
-        const int coord = ax == 0 ? x : (ax == 1 ? y : z);
-        const int n = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
-        if (coord != ((f & 1) ? n - 1 : 0)) continue;
-        if (k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM) {
-          if (from_lane && ax == 0) neem_correct<Q, ST>(a, f, h);
-          else neem_face<Q, ST, FLAGS>(a, x, y, z, f, h);
-        }
-        else
-          zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
-                       R(a.face[f].u[2]), R(a.face[f].rho));
-        break;
+    if (ff >= 0) {
+      // the whole-cell rule of face ff
+      const int f = ff, k = a.face[f].kind, ax = f >> 1;
+      if (k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM) {
+        if (from_lane && ax == 0) neem_correct<Q, ST>(a, f, h);
+        else neem_face<Q, ST, FLAGS>(a, x, y, z, f, h);
+      } else if (k == LBM_FACE_VELOCITY_ZH || k == LBM_FACE_PRESSURE_ZH) {
+        zou_he<Q, R>(h, ax, (f & 1) ? 1 : -1, k == LBM_FACE_VELOCITY_ZH, R(a.face[f].u[0]), R(a.face[f].u[1]),
+                     R(a.face[f].u[2]), R(a.face[f].rho));
+      } else {
+        wet_wall<Q, ST, FLAGS, W64>(a, x, y, z, f, h, missing, ppm, hold, cell);
       }
     }
 

@@ -223,6 +223,7 @@ struct lbm_handle {
   uint32_t *links = nullptr;  // solid-link masks (padded layout)
   bool has_flags = false;
   FaceDev face[6];
+  FaceDev *dface = nullptr;  // device copy of face[] read by the kernels
   int per[3] = {1, 1, 1};
   double omega = 1.0, F[3] = {0, 0, 0};
   void *snap[2] = {nullptr, nullptr};
@@ -252,6 +253,12 @@ static int fail(lbm_handle *h, int code, const std::string &msg) {
       return fail((h), LBM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
   } while (0)
 
+static int upload_faces(lbm_handle *h) {
+  HIPCHK(h, hipMemcpyAsync(h->dface, h->face, sizeof(h->face), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
 static KArgs make_args(lbm_handle *h) {
   KArgs a;
   memset(&a, 0, sizeof(a));
@@ -262,7 +269,7 @@ static KArgs make_args(lbm_handle *h) {
   a.plane = h->plane;
   a.nx = h->nx; a.ny = h->ny; a.nz = h->nz; a.px = h->px;
   for (int i = 0; i < 3; ++i) a.per[i] = h->per[i];
-  for (int f = 0; f < 6; ++f) a.face[f] = h->face[f];
+  a.face = h->dface;
   a.omega = h->omega;
   for (int i = 0; i < 3; ++i) a.F[i] = h->F[i];
   return a;
@@ -399,6 +406,10 @@ int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out)
     fail(nullptr, LBM_EHIP, "lbm_create: hipStreamCreate failed");
     return bail(LBM_EHIP);
   }
+  if (hipMalloc(&h->dface, sizeof(h->face)) != hipSuccess || upload_faces(h) != LBM_OK) {
+    fail(nullptr, LBM_EHIP, "lbm_create: face table upload failed");
+    return bail(LBM_EHIP);
+  }
   const size_t bytes = (size_t)h->plane * h->Q * es;
   for (int b = 0; b < 2; ++b) {
     if (hipMalloc(&h->buf[b], bytes) != hipSuccess) {
@@ -467,7 +478,10 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const lbm_bc_desc &d = bcs[i];
     if (d.face < 0 || d.face >= 2 * h->dim) return fail(h, LBM_EINVAL, "lbm_set_bc: face index out of range");
     if (seen[d.face]) return fail(h, LBM_EINVAL, "lbm_set_bc: face described twice");
-    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_PERIODIC_PRESSURE) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
+    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_WALL_NEBB) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
+    if (d.kind == LBM_FACE_WALL_NEBB && h->Q != 9) return fail(h, LBM_EINVAL, "lbm_set_bc: NEBB walls are D2Q9 only");
+    if (d.kind == LBM_FACE_WALL_NEBB && d.value[d.face >> 1] != 0.0)
+      return fail(h, LBM_EINVAL, "lbm_set_bc: a NEBB wall velocity must be tangential");
     seen[d.face] = true;
     face[d.face].kind = d.kind;
     face[d.face].prio = d.priority;
@@ -483,13 +497,30 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const int na = ax == 0 ? h->nx : (ax == 1 ? h->ny : h->nz);
     for (int s = 0; s < 2; ++s) {
       const int k = face[2 * ax + s].kind;
-      if ((k == LBM_FACE_PRESSURE_ABB || k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM) && na < 3)
+      if ((k == LBM_FACE_PRESSURE_ABB || k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM ||
+           k == LBM_FACE_WALL_NEEM) && na < 3)
         return fail(h, LBM_EINVAL, "lbm_set_bc: anti-bounce-back and NEEM faces need at least 3 cells along their normal");
+      if ((k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEBB) && na < 2)
+        return fail(h, LBM_EINVAL, "lbm_set_bc: wet-node walls need at least 2 cells along their normal");
     }
   }
   if (!neem_neighbours_ok(face, h->dim))
     return fail(h, LBM_EINVAL, "lbm_set_bc: the faces next to a NEEM face must be walls or periodic");
+  // a NEEM wall reads its inward neighbour's post-streaming populations,
+  // which are defined across periodic, periodic-pressure, bounce-back and
+  // anti-bounce-back faces (the latter bounced back plainly)
+  for (int f = 0; f < 2 * h->dim; ++f) {
+    if (face[f].kind != LBM_FACE_WALL_NEEM) continue;
+    for (int g = 0; g < 2 * h->dim; ++g) {
+      const int k = face[g].kind;
+      if ((g >> 1) != (f >> 1) && k != LBM_FACE_PERIODIC && k != LBM_FACE_PERIODIC_PRESSURE && k != LBM_FACE_WALL &&
+          k != LBM_FACE_PRESSURE_ABB)
+        return fail(h, LBM_EINVAL, "lbm_set_bc: the faces next to a NEEM wall must be periodic, bounce-back or anti-bounce-back");
+    }
+  }
   memcpy(h->face, face, sizeof(face));
+  const int urc = upload_faces(h);
+  if (urc) return urc;
   memcpy(h->per, per, sizeof(per));
   return build_links(h);
 }
@@ -798,6 +829,7 @@ int lbm_destroy(lbm_handle *h) {
     if (h->snap[b]) hipFree(h->snap[b]);
   if (h->partials) hipFree(h->partials);
   if (h->sums_d) hipFree(h->sums_d);
+  if (h->dface) hipFree(h->dface);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return LBM_OK;

@@ -31,8 +31,12 @@ void LBMBndManager::setupBndryCnds(const json& bndConfig, CartesianGrid& grid) {
         const auto model = required_value<std::string>(props, "model");
         if (model == "bounceback") {
           m_bndrys.emplace_back(std::make_unique<LBMBnd_wallBB>(&srf, props, dim));
-        } else if (model == "equilibrium" || model == "neem" || model == "nebb") {
-          TERMM(-1, "Wall boundary model '" + model + "' (wet node) is not part of this build");
+        } else if (model == "equilibrium") {
+          m_bndrys.emplace_back(std::make_unique<LBMBnd_wallWetnode>(&srf, props, dim, LBM_FACE_WALL_EQ));
+        } else if (model == "neem") {
+          m_bndrys.emplace_back(std::make_unique<LBMBnd_wallWetnode>(&srf, props, dim, LBM_FACE_WALL_NEEM));
+        } else if (model == "nebb") {
+          m_bndrys.emplace_back(std::make_unique<LBMBnd_wallWetnode>(&srf, props, dim, LBM_FACE_WALL_NEBB));
         } else {
           TERMM(-1, "Invalid wall boundary model: " + model);
         }

@@ -145,6 +145,43 @@ class LBMBnd_wallBB : public LBMBndInterface {
   GDouble m_tangentialV = 0;
 };
 
+// Wet-node walls: 'wall' with model 'equilibrium' (LBMBnd_wallEq,
+// bnd_wall.h:100-185), 'neem' (LBMBnd_wallNEEM, bnd_wall.h:187-300) or
+// 'nebb' (LBMBnd_wallNEBB, bnd_wall.h:303-479, D2Q9 only); optional
+// 'velocity' (no-slip without it).  initCnd is empty, as in the reference.
+class LBMBnd_wallWetnode : public LBMBndInterface {
+ public:
+  LBMBnd_wallWetnode(const Surface* s, const json& props, GInt dim, int kind) : m_s(s), m_kind(kind) {
+    if (kind == LBM_FACE_WALL_NEBB && dim != 2) TERMM(-1, "Not implemented for this distribution!");
+    if (props.count("velocity")) {
+      auto v = required_value<std::vector<GDouble>>(props, "velocity");
+      if ((GInt)v.size() < dim) TERMM(-1, "wall velocity needs dim entries");
+      for (GInt d = 0; d < dim; ++d) m_value[d] = v[d];
+    }
+  }
+  void initCnd(const VarAccess&) override {}
+  [[nodiscard]] auto lower(GInt cellId) const -> BndRule override {
+    BndRule r;
+    r.kind = m_kind;
+    r.value = m_value;
+    if (m_kind == LBM_FACE_WALL_NEBB) {
+      // the reference reflects the normal population with the normal
+      // velocity of the cell; this build takes the wall's, so only
+      // tangential wall velocities are accepted
+      const GDouble* n = m_s->normal_p(cellId);
+      for (GInt d = 0; d < 3; ++d)
+        if (n[d] != 0 && m_value[d] != 0) TERMM(-1, "A NEBB wall velocity must be tangential to the wall");
+    }
+    return r;
+  }
+  [[nodiscard]] auto surface() const -> const Surface* override { return m_s; }
+
+ private:
+  const Surface* m_s;
+  int m_kind;
+  std::array<GDouble, 3> m_value{{0, 0, 0}};
+};
+
 // 'pressure' anti-bounce-back (bnd_pressure.h:11-113); 3D form new.
 class LBMBnd_Pressure : public LBMBndInterface {
  public:

@@ -75,7 +75,8 @@ void CartesianGrid::generate(Configuration& cfg) {
   cfg.opt_config_value<std::string>("outputDir", "out");
   cfg.opt_config_value<std::string>("gridFileName", "grid");
   cfg.getObject("output");
-  cfg.opt_config_value<bool>("alignNodesWithSurface", false);
+  const bool align = cfg.opt_config_value<bool>("alignNodesWithSurface", false);
+  const GInt alignDir = cfg.opt_config_value<GInt>("alignDir", 1);
 
   m_geom = std::make_shared<GeometryManager>();
   json defGeom = {{"cube", {{"type", "cube"}, {"center", {0.0, 0.0, 0.0}}, {"length", 1}}}};
@@ -261,6 +262,7 @@ void CartesianGrid::generate(Configuration& cfg) {
     m_cell[q] = cells[order[q]];
     centreOf(m_level, m_cell[q][0], m_cell[q][1], m_cell[q][2], &m_center[q * 3]);
   }
+  if (align) alignWithSurface(alignDir);
   // structured window and main neighbours
   m_lo = w.lo;
   m_n = w.n;
@@ -280,6 +282,47 @@ void CartesianGrid::generate(Configuration& cfg) {
       if (!w.inside(co[0], co[1], co[2])) continue;
       m_nghbr[q * m_stride + dir] = m_cellOf[w.idx(co[0], co[1], co[2])];
     }
+  }
+}
+
+// transformMaxRfnmtLvlToExtent (src/gridgenerator/cartesiangrid_generation.h:256-304):
+// stretch the cell centres so that the outermost ones lie on the bounding
+// box along alignDir (along every axis when the nodal extent is the same in
+// all of them) and scale the cell length alike.  The reference then deletes
+// the cells that are neither cut nor inside; that would reshape the
+// structured window, so such a configuration is rejected here.
+void CartesianGrid::alignWithSurface(GInt alignDir) {
+  const GInt N = (GInt)m_cell.size();
+  if (alignDir < 0 || alignDir >= m_dim) TERMM(-1, "Invalid alignDir");
+  std::array<GDouble, 3> mn{{std::numeric_limits<GDouble>::max(), std::numeric_limits<GDouble>::max(),
+                             std::numeric_limits<GDouble>::max()}};
+  std::array<GDouble, 3> mx{{std::numeric_limits<GDouble>::min(), std::numeric_limits<GDouble>::min(),
+                             std::numeric_limits<GDouble>::min()}};
+  for (GInt q = 0; q < N; ++q)
+    for (int d = 0; d < m_dim; ++d) {
+      if (mn[d] > m_center[q * 3 + d]) mn[d] = m_center[q * 3 + d];
+      if (mx[d] < m_center[q * 3 + d]) mx[d] = m_center[q * 3 + d];
+    }
+  bool same = true;
+  for (int d = 0; d < m_dim && same; ++d)
+    for (int d2 = d + 1; d2 < m_dim; ++d2)
+      if (!(std::abs(mn[d] - mn[d2]) < std::numeric_limits<GDouble>::epsilon() &&
+            std::abs(mx[d] - mx[d2]) < std::numeric_limits<GDouble>::epsilon())) {
+        same = false;
+        break;
+      }
+  const GDouble T = m_bbox.max[alignDir] / (mx[alignDir] - mn[alignDir]);
+  for (GInt q = 0; q < N; ++q)
+    for (int d = 0; d < m_dim; ++d) {
+      GDouble& c = m_center[q * 3 + d];
+      if (d == alignDir || same) c = c * T - (T * mx[d] - m_bbox.max[d]);
+      else c = c * T;
+    }
+  m_len[m_level] *= T;
+  for (GInt q = 0; q < N; ++q) {
+    const GDouble* c = &m_center[q * 3];
+    if (!m_geom->cutWithCell(c, m_len[m_level]) && !m_geom->pointIsInside(c))
+      TERMM(-1, "alignNodesWithSurface moves cells out of the geometry; deleting them is not supported by this build");
   }
 }
 

@@ -101,6 +101,8 @@ class CartesianGrid : public GridInterface {
   // Grid generator part: keys dim, maxNoCells, partitionLevel, uniformLevel,
   // maxRfnmtLvl, geometry, boundingBox (src/gridgenerator/gridGenerator.cpp:107-286).
   void generate(Configuration& cfg);
+  // transformMaxRfnmtLvlToExtent for alignNodesWithSurface
+  void alignWithSurface(GInt alignDir);
   // Solver part (CartesianGrid::loadGridInplace, src/cartesiangrid.h:264-320):
   // boundary cells, surfaces from solver.boundary, periodic links, diagonals.
   void loadGridInplace(Configuration& solverCfg);

@@ -17,11 +17,13 @@ grid), :343-380 (refineGrid), :419-451 (refineCell), :453-506 (neighbours),
 src/geometry.h:640-1075 (box/cube/sphere tests); include/common/math/cartesian.h:72-153.
 """
 import math
+import sys
 
 from oracle import oracle as O
 
 EPS = 2.220446049250313e-16
 MAX_LVL = 100
+EPS_DBL = sys.float_info.epsilon  # approx(a, b), include/common/math/mathfunctions.h:60-62
 
 # include/common/math/cartesian.h:72-101 (first 8 children)
 CHILD_DIR = [(-1, -1, -1), (1, -1, -1), (-1, 1, -1), (1, 1, -1), (-1, -1, 1), (1, -1, 1), (-1, 1, 1), (1, 1, 1)]
@@ -158,6 +160,8 @@ class Generator:
         self.L = cfg["partitionLevel"]
         assert cfg["uniformLevel"] == self.L and cfg.get("maxRfnmtLvl", self.L) == self.L
         self.geom = Geometry(cfg["geometry"], self.dim)
+        self.align = bool(cfg.get("alignNodesWithSurface", False))
+        self.align_dir = int(cfg.get("alignDir", 1))
         mn, mx = self.geom.bbox(self.dim)
         ext = [abs(mx[d] - mn[d]) for d in range(self.dim)]
         decisive = 0
@@ -211,7 +215,50 @@ class Generator:
         self.size = self.offsets[self.L][1] - self.offsets[self.L][0]
         assert self.offsets[self.L][0] == 0
         self.reorder_hilbert()
+        if self.align:
+            self.align_with_surface(self.align_dir)
         return self
+
+    def align_with_surface(self, align_dir):
+        """transformMaxRfnmtLvlToExtent
+        (src/gridgenerator/cartesiangrid_generation.h:256-304): stretch the
+        finest level so that its outermost cell centres lie on the bounding
+        box along align_dir (along every axis when the nodal extent is the
+        same in all of them), scale the cell length alike, then delete the
+        cells that are neither cut nor inside."""
+        b, e = self.offsets[self.L]
+        mn = [sys.float_info.max] * self.dim
+        mx = [sys.float_info.min] * self.dim
+        for c in range(b, e):
+            for d in range(self.dim):
+                if mn[d] > self.center[c][d]:
+                    mn[d] = self.center[c][d]
+                if mx[d] < self.center[c][d]:
+                    mx[d] = self.center[c][d]
+        same = True
+        for d in range(self.dim):
+            for d2 in range(d + 1, self.dim):
+                if not (abs(mn[d] - mn[d2]) < EPS_DBL and abs(mx[d] - mx[d2]) < EPS_DBL):
+                    same = False
+            if not same:
+                break
+        bmin, bmax = self.geom.bbox(self.dim)
+        T = bmax[align_dir] / (mx[align_dir] - mn[align_dir])
+        for c in range(b, e):
+            for d in range(self.dim):
+                if d == align_dir or same:
+                    self.center[c][d] = self.center[c][d] * T - (T * mx[d] - bmax[d])
+                else:
+                    self.center[c][d] = self.center[c][d] * T
+        self.len[self.L] *= T
+        for c in range(b, e):
+            isb = self.geom.cut(self.center[c], self.len[self.L])
+            self.bndry[c] = isb
+            self.inside[c] = isb or self.geom.inside(self.center[c])
+        for c in range(e - 1, b - 1, -1):
+            if not self.inside[c]:
+                self.delete_cell(c)
+        self.size = self.offsets[self.L][1] - self.offsets[self.L][0]
 
     def refine_cell(self, cid, off):
         lvl = self.level[cid] + 1

@@ -15,6 +15,9 @@
  *   - push propagation through neighbour table . src/lbm/solver.cpp:715-740
  *   - halfway bounce-back (+Ladd wall velocity)  src/lbm/bnd/bnd_dirichlet.h:79-121
  *   - anti-bounce-back pressure ................ src/lbm/bnd/bnd_pressure.h:55-106
+ *   - periodic pair with a density jump ........ src/lbm/bnd/bnd_periodic.h:102-108
+ *   - wet-node walls (equilibrium, NEEM, NEBB) . src/lbm/bnd/bnd_wall.h:100-479,
+ *     limited density ........................... bnd_wetnode.h:20-55, moments.h:120-153
  *   - residual sum |q^n - q^(n-1)| ............. src/lbm/solver.cpp:233-263,808-815
  *   - Hilbert index ............................ include/common/math/hilbert.h:15-48
  * New components with no reference counterpart (pinned by closed forms and
@@ -159,8 +162,19 @@ int orc_macros(int Q, int64_t ncell, const double *fold, const double *force, do
  *          (bnd_periodic.h:102-108): fold(c, o) = feq_o(rho, u_n1) +
  *          fpost(n1, o) - feq_o(vars(n1)) for the missing directions, n1 the
  *          linked cell on the other face, rho the pressure of n1's surface
+ *   8 EQW  wet-node equilibrium wall, LBMBnd_wallEq (bnd_wall.h:100-185)
+ *          through LBMBnd_DirichletEQ::apply (bnd_dirichlet.h:203-236)
+ *   9 NEEMW wet-node non-equilibrium extrapolation wall, LBMBnd_wallNEEM
+ *          (bnd_wall.h:187-300): EQW, then + fold(n1) - feq(moments of n1)
+ *  10 NEBBW wet-node non-equilibrium bounce-back wall, D2Q9, LBMBnd_wallNEBB
+ *          apply_constV (bnd_wall.h:404-472)
  * dirmask bit i: apply to lattice direction i (the direction of the missing
- * neighbour), only consulted by BB and ABB. */
+ * neighbour), only consulted by BB, ABB and PP.
+ * Wet-node walls: lim1 / lim2 bit i = population i enters the limited
+ * density with constant 1 / 2 (LBMBnd_wallWetnode, bnd_wetnode.h:20-55; the
+ * rest population is a lim1 bit); flags bit 0: the wall has a velocity
+ * (apply_constV), bit 1: empty limited set (a corner, density unchanged);
+ * axis/sign: the outward normal. */
 typedef struct {
   int64_t cell;
   int32_t kind;
@@ -170,7 +184,24 @@ typedef struct {
   int64_t n1, n2;
   double u[3];
   double rho;
+  int32_t lim1, lim2;
+  int32_t flags, pad;
 } orc_bc;
+
+/* calcDensity_limited (src/lbm/moments.h:120-153): rho = sum over the
+ * limited set (ascending distribution order) of const_i fold(c, i); with a
+ * wall velocity rho *= 1 / (1 + n.u) along the normal, u the velocity the
+ * cell holds in vars at that moment.  An empty set leaves vars untouched. */
+static void wet_density(const lattice_t *L, const orc_bc *r, const double *f, double *v) {
+  if (r->flags & 2) return;
+  double rho = 0;
+  for (int i = 0; i < L->Q; ++i) {
+    if ((r->lim2 >> i) & 1) rho += 2.0 * f[i];
+    else if ((r->lim1 >> i) & 1) rho += 1.0 * f[i];
+  }
+  if (r->flags & 1) rho *= 1.0 / (1.0 + (double)r->sign * v[r->axis]);
+  v[L->dim] = rho;
+}
 
 /* Zou-He with transverse-momentum correction (Hecht & Harting, J. Stat.
  * Mech. 2010 P01018), written for any face normal of D2Q9/D3Q19:
@@ -309,6 +340,72 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
         for (int d = 0; d < L.dim; ++d) cu += L.c[o][d] * v1[d];
         fold[c * Q + o] = default_eq(L.w[o], r->rho, cu, vsq) + fpost[r->n1 * Q + o] -
                           default_eq(L.w[o], v1[L.dim], cu, vsq);
+      }
+    } else if (r->kind == 8 || r->kind == 9) {
+      /* DirichletEQ::apply: the wall velocity into vars, the limited
+       * density, then every population = feq(rho, u_w) (w_i rho without a
+       * velocity, VALZERO) */
+      double *v = vars + c * nv;
+      for (int d = 0; d < L.dim; ++d) v[d] = r->u[d];
+      wet_density(&L, r, fold + c * Q, v);
+      const double rho = v[L.dim];
+      if (r->flags & 1) {
+        feq_all(&L, rho, r->u, fold + c * Q);
+      } else {
+        for (int i = 0; i < Q; ++i) fold[c * Q + i] = default_eq(L.w[i], rho, 0.0, 0.0);
+      }
+      if (r->kind == 9) {
+        /* apply_general: moments of the extrapolation cell n1 (calcDensity /
+         * calcVelocity, moments.h:17-101: no force term), then
+         * fold(c, i) += fold(n1, i) - feq_i(rho1, u1).  The reference
+         * also stores rho1, u1 in vars(n1); later objects of the same
+         * step would read them there.  This restatement keeps them local. */
+        const double *f1 = fold + r->n1 * Q;
+        double rho1 = f1[0];
+        for (int i = 1; i < Q; ++i) rho1 += f1[i];
+        double u1[3] = {0, 0, 0};
+        for (int d = 0; d < L.dim; ++d) {
+          for (int i = 0; i < Q - 1; ++i) u1[d] += L.c[i][d] * f1[i];
+          u1[d] /= rho1;
+        }
+        double vsq = 0;
+        for (int d = 0; d < L.dim; ++d) vsq += u1[d] * u1[d];
+        for (int i = 0; i < Q; ++i) {
+          double cu = 0;
+          for (int d = 0; d < L.dim; ++d) cu += u1[d] * L.c[i][d];
+          fold[c * Q + i] += f1[i] - default_eq(L.w[i], rho1, cu, vsq);
+        }
+      }
+    } else if (r->kind == 10) {
+      /* NEBB (D2Q9): limited density with the velocity the cell holds, then
+       * the wall velocity into vars; the normal population reflected with
+       * -/+ 2/3 rho u_n; the two unknown diagonals from the known ones,
+       * the tangential difference and the wall velocity (Zou & He 1997). */
+      if (Q != 9) return -3;
+      double *v = vars + c * nv;
+      double *f = fold + c * Q;
+      wet_density(&L, r, f, v);
+      for (int d = 0; d < L.dim; ++d) v[d] = r->u[d];
+      const double rho = v[L.dim];
+      const double *uw = r->u;
+      const int out = 2 * r->axis + (r->sign > 0 ? 1 : 0); /* outward main direction */
+      if (nbr[c * (Q - 1) + out] < 0) {
+        f[L.opp[out]] = f[out];
+        if (r->sign < 0) f[L.opp[out]] -= 2.0 / 3.0 * rho * uw[r->axis];
+        else f[L.opp[out]] += 2.0 / 3.0 * rho * uw[r->axis];
+      }
+      if (r->axis == 0 && r->sign > 0) {
+        f[6] = f[4] + 0.5 * (f[3] - f[2]) - 0.5 * rho * uw[1] - 1.0 / 6.0 * rho * uw[0];
+        f[7] = f[5] - 0.5 * (f[3] - f[2]) + 0.5 * rho * uw[1] - 1.0 / 6.0 * rho * uw[0];
+      } else if (r->axis == 0) {
+        f[4] = f[6] - 0.5 * (f[3] - f[2]) + 0.5 * rho * uw[1] + 1.0 / 6.0 * rho * uw[0];
+        f[5] = f[7] + 0.5 * (f[3] - f[2]) - 0.5 * rho * uw[1] + 1.0 / 6.0 * rho * uw[0];
+      } else if (r->sign > 0) {
+        f[5] = f[7] - 0.5 * (f[1] - f[0]) + 0.5 * rho * uw[0] - 1.0 / 6.0 * rho * uw[1];
+        f[6] = f[4] + 0.5 * (f[1] - f[0]) - 0.5 * rho * uw[0] - 1.0 / 6.0 * rho * uw[1];
+      } else {
+        f[7] = f[5] + 0.5 * (f[1] - f[0]) - 0.5 * rho * uw[0] + 1.0 / 6.0 * rho * uw[1];
+        f[4] = f[6] - 0.5 * (f[1] - f[0]) + 0.5 * rho * uw[0] + 1.0 / 6.0 * rho * uw[1];
       }
     } else if (r->kind == 5 || r->kind == 6) {
       /* NEEM faces (Guo, Zheng & Shi, Chin. Phys. 11 366 (2002)): every

@@ -24,13 +24,48 @@ LIB = os.path.join(HERE, "liblbm_oracle.so")
 REF_HILBERT = os.path.join(HERE, "_ref", "libref_hilbert.so")
 
 FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6,
-        "periodic_pressure": 7}
+        "periodic_pressure": 7, "wall_eq": 8, "wall_neem": 9, "wall_nebb": 10}
+# faces whose rule replaces the populations of the whole face cell
+FULL = (3, 4, 5, 6, 8, 9, 10)
+WET = (8, 9, 10)
+EPS = 2.220446049250313e-16
 
 
 class OrcBC(ct.Structure):
     _fields_ = [("cell", ct.c_int64), ("kind", ct.c_int32), ("dirmask", ct.c_int32), ("axis", ct.c_int32),
                 ("sign", ct.c_int32), ("n1", ct.c_int64), ("n2", ct.c_int64), ("u", ct.c_double * 3),
-                ("rho", ct.c_double)]
+                ("rho", ct.c_double), ("lim1", ct.c_int32), ("lim2", ct.c_int32), ("flags", ct.c_int32),
+                ("pad", ct.c_int32)]
+
+
+def limited_set(c, nrm, Q, upstream_ok, downstream_missing):
+    """Limited density constants of a wet-node wall cell
+    (LBMBnd_wallWetnode, src/lbm/bnd/bnd_wetnode.h:20-55): population i with
+    c_i.n >= eps (pointing out through the wall) counts twice, one with
+    |c_i.n| <= eps once (twice when its downstream neighbour is missing and
+    the cell is not periodic), each only when its upstream cell exists;
+    if the constants do not add up to Q-1 the set is empty (a corner).
+    upstream_ok(i), downstream_missing(i) -> bool.  Returns (lim1, lim2,
+    empty)."""
+    lim1 = lim2 = 0
+    total = 0
+    for i in range(Q - 1):
+        cn = float(c[i] @ nrm)
+        if not upstream_ok(i):
+            continue
+        if cn >= EPS:
+            lim2 |= 1 << i
+            total += 2
+        elif abs(cn) <= EPS:
+            if downstream_missing(i):
+                lim2 |= 1 << i
+                total += 2
+            else:
+                lim1 |= 1 << i
+                total += 1
+    if total != Q - 1:
+        return 0, 0, True
+    return lim1 | (1 << (Q - 1)), lim2, False
 
 
 _lib = None
@@ -168,8 +203,21 @@ class Problem:
         self.fluid = ~self.solid
         self.nbr = nbr
         recs = []
+        wet = []
+        after = []
         zh = []
+        PP = FACE["periodic_pressure"]
         for cidx in np.nonzero(self.fluid & (nbr < 0).any(axis=1))[0]:
+            co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+            # the whole-cell rule: the full-cell face of highest priority the
+            # cell lies on (ties to the lower face index)
+            F = None
+            for f in range(2 * dim):
+                ax, side = f // 2, f % 2
+                if self.face_kind[f] in FULL and co[ax] == (ext[ax] - 1 if side else 0):
+                    if F is None or self.face_prio[f] > self.face_prio[F]:
+                        F = f
+            wetF = F is not None and self.face_kind[F] in WET
             groups = {}
             for i in range(Q - 1):
                 if nbr[cidx, i] >= 0:
@@ -188,10 +236,13 @@ class Problem:
                     continue
                 kind = self.face_kind[key]
                 ax, side = key // 2, key % 2
+                # a wet-node wall replaces the cell, except the populations
+                # whose face has the higher priority: those are written after
+                dest = after if wetF and self.face_prio[key] > self.face_prio[F] else recs
                 if kind == FACE["wall"]:
                     r.kind = 1
                     r.u = (ct.c_double * 3)(*self.face_u[key])
-                    recs.append(r)
+                    dest.append(r)
                 elif kind == FACE["pressure_abb"]:
                     r.kind = 2
                     r.rho = self.face_rho[key]
@@ -201,14 +252,14 @@ class Problem:
                     c2 = list(co); c2[ax] += 2 * step
                     r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
                     r.n2 = c2[0] + nx * (c2[1] + ny * c2[2])
-                    recs.append(r)
+                    dest.append(r)
                 elif kind == FACE["periodic_pressure"]:
                     r.kind = 7
                     r.rho = self.face_rho[key ^ 1]  # pressure of the other face
                     c1 = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
                     c1[ax] = 0 if side == 1 else ext[ax] - 1
                     r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
-                    recs.append(r)
+                    dest.append(r)
                 elif kind in (FACE["pressure_neem"], FACE["velocity_neem"]):
                     # the face record (below) replaces every population; with
                     # a solid inward neighbour the missing ones bounce back
@@ -218,15 +269,40 @@ class Problem:
                     if self.solid[c1[0] + nx * (c1[1] + ny * c1[2])]:
                         r.kind = 1
                         recs.append(r)
-            # Zou-He / NEEM: first such face the cell lies on
-            co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
-            for f in range(2 * dim):
+            if wetF:
+                # wet-node wall: limited density over the populations that
+                # stream in (or enter across a periodic-pressure face)
+                ax, side = F // 2, F % 2
+                nrm = np.zeros(3)
+                nrm[ax] = 1.0 if side else -1.0
+
+                def pp_crossed(i, cidx=cidx):
+                    fc = int(crossed_face[cidx, i])
+                    return fc >= 0 and self.face_kind[fc] == PP
+
+                lim1, lim2, empty = limited_set(
+                    c, nrm, Q, lambda i: nbr[cidx, opp[i]] >= 0 or pp_crossed(opp[i]),
+                    lambda i: nbr[cidx, i] < 0 and not pp_crossed(i))
+                r = OrcBC()
+                r.cell = int(cidx)
+                r.kind = self.face_kind[F]
+                r.axis = ax
+                r.sign = 1 if side else -1
+                r.lim1, r.lim2 = lim1, lim2
+                # the mass balance divides by 1 + n.u_w; NEBB walls take a
+                # tangential velocity, so no division (the reference divides
+                # by the cell's own normal velocity there, zero once the wall
+                # has acted)
+                r.flags = (0 if r.kind == 10 else 1) | (2 if empty else 0)
+                r.u = (ct.c_double * 3)(*self.face_u[F])
+                c1 = list(co)
+                c1[ax] += -1 if side else 1
+                r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
+                wet.append(r)
+            # Zou-He / NEEM face
+            for f in ([F] if F is not None and not wetF else []):
                 k = self.face_kind[f]
-                if k not in (FACE["velocity_zh"], FACE["pressure_zh"], FACE["pressure_neem"], FACE["velocity_neem"]):
-                    continue
                 ax, side = f // 2, f % 2
-                if co[ax] != (ext[ax] - 1 if side else 0):
-                    continue
                 r = OrcBC()
                 r.cell = int(cidx)
                 r.kind = {FACE["velocity_zh"]: 3, FACE["pressure_zh"]: 4, FACE["pressure_neem"]: 5,
@@ -243,7 +319,7 @@ class Problem:
                 r.rho = self.face_rho[f]
                 zh.append(r)
                 break
-        allr = recs + zh
+        allr = recs + wet + after + zh
         self.nrec = len(allr)
         self.recs = (OrcBC * max(1, len(allr)))(*allr)
 
@@ -380,6 +456,12 @@ class GridProblem(Problem):
         pre = []  # periodic pressure records: the reference's preApply runs before every apply
         gated = np.zeros((n, self.Q - 1), dtype=bool)
         self.init_u = {}
+        # CellProperties::periodic: cells of 'periodic' objects that are
+        # generated (bnd.h:219-220)
+        periodic_cell = np.zeros(n, dtype=bool)
+        for name, spec in bcs:
+            if spec["type"] == "periodic" and spec.get("generateBndry", True):
+                periodic_cell[np.asarray(surfaces[name][0], dtype=np.int64)] = True
         for name, spec in bcs:
             cells, dirs = surfaces[name]
             t = spec["type"]
@@ -421,7 +503,27 @@ class GridProblem(Problem):
                 r = OrcBC()
                 r.cell = int(cidx)
                 r.dirmask = mask
-                if t == "wall":
+                if t == "wall" and spec["model"] in ("equilibrium", "neem", "nebb"):
+                    # wet-node walls (bnd_wall.h:100-479) in manager order
+                    nrm3 = np.zeros(3)
+                    nrm3[:dim] = nrm[:dim]
+                    per = bool(periodic_cell[cidx])
+                    lim1, lim2, empty = limited_set(
+                        c, nrm3, self.Q, lambda i: per or nbr_table[cidx, opp[i]] >= 0,
+                        lambda i: nbr_table[cidx, i] < 0 and not per)
+                    r.kind = {"equilibrium": 8, "neem": 9, "nebb": 10}[spec["model"]]
+                    a = d // 2
+                    r.axis = a
+                    r.sign = 1 if d % 2 else -1
+                    r.lim1, r.lim2 = lim1, lim2
+                    r.flags = (1 if "velocity" in spec else 0) | (2 if empty else 0)
+                    v = list(spec.get("velocity", [0.0] * dim))[:dim] + [0.0] * (3 - dim)
+                    r.u = (ct.c_double * 3)(*v)
+                    # extrapolation cell: the neighbour against the normal
+                    # (LBMBnd_wallNEEM::init, bnd_wall.h:213-245)
+                    r.n1 = int(nbr_table[cidx, 2 * a + 1 if nrm[a] < 0 else 2 * a])
+                    recs.append(r)
+                elif t == "wall":
                     r.kind = 1
                     V = spec.get("tangentialVelocity", 0.0)
                     r.u = (ct.c_double * 3)(V * nrm[1], V * nrm[0], 0.0)

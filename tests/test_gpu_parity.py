@@ -365,3 +365,51 @@ def test_periodic_pressure_channel(storage, tol, shape):
     eu, er, _ = run_pair(shape, 150, storage, 1 / 0.7, faces=faces, seed=12)
     print(f"periodic pressure {shape} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
     assert eu < tol and er < tol
+
+
+# wet-node walls (equilibrium / NEEM / NEBB, src/lbm/bnd/bnd_wall.h:100-479)
+WET_CASES = {
+    "eq_neem_channel2d": ((48, 20), {"-y": ("wall_eq", None, 1.0), "+y": ("wall_neem", (0.05, 0.0), 1.0)}),
+    "nebb_channel2d": ((40, 16), {"-y": ("wall_nebb", (0.02, 0.0), 1.0), "+y": ("wall_nebb", (-0.03, 0.0), 1.0)}),
+    "pp_eq_neem2d": ((40, 14), {"-x": ("periodic_pressure", None, 1.002), "+x": ("periodic_pressure", None, 1.0),
+                                "-y": ("wall_eq", None, 1.0), "+y": ("wall_neem", None, 1.0)}),
+    "pp_nebb2d": ((36, 12), {"-x": ("periodic_pressure", None, 1.002), "+x": ("periodic_pressure", None, 1.0),
+                             "-y": ("wall_nebb", None, 1.0), "+y": ("wall_nebb", None, 1.0)}),
+    # corners: anti-bounce-back x faces below / above the walls in priority
+    "abb_eq_low2d": ((40, 14), {"-x": ("pressure_abb", None, 1.002, 1), "+x": ("pressure_abb", None, 1.0, 1),
+                                "-y": ("wall_eq", None, 1.0, 2), "+y": ("wall_eq", None, 1.0, 2)}),
+    "abb_eq_mixed2d": ((40, 14), {"-x": ("pressure_abb", None, 1.002, 3), "+x": ("pressure_abb", None, 1.0, 1),
+                                  "-y": ("wall_eq", None, 1.0, 4), "+y": ("wall_eq", None, 1.0, 2)}),
+    "box_bb_eq2d": ((30, 12), {"-x": ("wall", None, 1.0, 3), "+x": ("wall", None, 1.0, 1),
+                               "-y": ("wall_eq", (0.03, 0.0), 1.0, 2), "+y": ("wall_neem", None, 1.0, 2)}),
+    "eq_neem3d": ((24, 10, 8), {"-y": ("wall_eq", (0.0, 0.0, 0.01), 1.0), "+y": ("wall_neem", (0.04, 0.0, 0.0), 1.0)}),
+    "edges3d": ((20, 10, 9), {"-y": ("wall_eq", None, 1.0, 2), "+y": ("wall_neem", (0.03, 0.0, 0.0), 1.0, 2),
+                              "-z": ("wall", None, 1.0, 3), "+z": ("wall", None, 1.0, 1)}),
+    "pp_neem3d": ((20, 10, 6), {"-x": ("periodic_pressure", None, 1.002), "+x": ("periodic_pressure", None, 1.0),
+                                "-y": ("wall_neem", None, 1.0), "+y": ("wall_eq", None, 1.0)}),
+}
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f64", 1e-11)])
+@pytest.mark.parametrize("case", sorted(WET_CASES))
+def test_wet_node_walls(storage, tol, case):
+    shape, faces = WET_CASES[case]
+    eu, er, _ = run_pair(shape, 200, storage, 1 / 0.7, faces=faces, seed=21)
+    print(f"wet-node {case} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < tol and er < tol
+
+
+def test_wet_node_wall_validation():
+    lat = capi.Lattice((10, 8, 6), storage="f32", omega=1.0)
+    with pytest.raises(capi.LBMError):  # NEBB is D2Q9 only
+        lat.set_faces({"-y": ("wall_nebb", None, 1.0), "+y": ("wall", None, 1.0)})
+    lat.close()
+    lat = capi.Lattice((12, 8), storage="f32", omega=1.0)
+    with pytest.raises(capi.LBMError):  # NEBB velocity must be tangential
+        lat.set_faces({"-y": ("wall_nebb", (0.01, 0.02), 1.0), "+y": ("wall", None, 1.0)})
+    with pytest.raises(capi.LBMError):  # NEEM wall next to a Zou-He face
+        lat.set_faces({"-x": ("velocity_zh", (0.02, 0.0), 1.0), "+x": ("pressure_zh", None, 1.0),
+                       "-y": ("wall_neem", None, 1.0), "+y": ("wall", None, 1.0)})
+    lat.set_faces({"-x": ("pressure_abb", None, 1.001), "+x": ("pressure_abb", None, 1.0),
+                   "-y": ("wall_neem", None, 1.0), "+y": ("wall_eq", None, 1.0)})
+    lat.close()

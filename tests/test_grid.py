@@ -71,6 +71,8 @@ CASES = [
     (REFCFG, "couette__couette.json"),
     (REFCFG, "couette__couette_bnd_bbDirichlet.json"),
     (REFCFG, "poiseuille__poiseuille_bnd_pressure.json"),
+    (REFCFG, "couette__couette_bnd_eq_aligned.json"),
+    (REFCFG, "poiseuille__poiseuille_bnd_NEEM.json"),
     (REFCFG, "step__step_ns.json"),
     (OWNCFG, "box3d_periodic_x.json"),
     (OWNCFG, "sphere3d_channel.json"),

@@ -52,6 +52,25 @@ def test_poiseuille_periodic_pressure_reference_thresholds_f64(tmp_path):
     assert "max. Error" in r.stderr
 
 
+WETNODE_REFCFG = ["couette__couette_bnd_eq.json", "couette__couette_bnd_eq2.json", "couette__couette_bnd_eq_aligned.json",
+                  "couette__couette_bnd_NEEM.json", "couette__couette_bnd_NEBB.json", "poiseuille__poiseuille_bnd_eq.json",
+                  "poiseuille__poiseuille_bnd_NEEM.json", "poiseuille__poiseuille_bnd_NEBB.json",
+                  "poiseuille__poiseuille_bnd_pressure.json", "poiseuille__poiseuille_bnd_pressure_neem2.json"]
+
+
+@pytest.mark.parametrize("name", WETNODE_REFCFG)
+def test_wetnode_reference_thresholds_f64(name, tmp_path):
+    """The reference's Couette and Poiseuille cases with wet-node walls
+    (equilibrium, NEEM, NEBB) through the CLI in FP64: each must pass its own
+    analytic error thresholds (exit status 0)."""
+    cfg = json.load(open(os.path.join(REFCFG, name)))
+    r = run_cli(cfg, tmp_path, "-p", "f64", timeout=600)
+    tail = "\n".join(l for l in r.stderr.splitlines() if "Error" in l or "error" in l or "convergence" in l)
+    print(name, r.returncode, tail[-1500:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "max. Error" in r.stderr
+
+
 def test_couette_reference_f32_fails_loudly_only_on_thresholds(tmp_path):
     """FP32 storage cannot reach the 4e-11 bound: the run must end through the
     analytic check with status 255, like the reference's TERMM."""
@@ -148,6 +167,15 @@ CASES = [
     ("neem_channel2d.json", OWNCFG, 400, False),
     ("sphere3d_neem.json", OWNCFG, 150, True),
     ("poiseuille__poiseuille_bnd.json", REFCFG, 300, False),
+    ("couette__couette_bnd_eq.json", REFCFG, 300, False),
+    ("couette__couette_bnd_eq2.json", REFCFG, 300, False),
+    ("couette__couette_bnd_eq_aligned.json", REFCFG, 300, False),
+    ("couette__couette_bnd_NEEM.json", REFCFG, 300, False),
+    ("couette__couette_bnd_NEBB.json", REFCFG, 300, False),
+    ("poiseuille__poiseuille_bnd_eq.json", REFCFG, 300, False),
+    ("poiseuille__poiseuille_bnd_NEEM.json", REFCFG, 300, False),
+    ("poiseuille__poiseuille_bnd_NEBB.json", REFCFG, 300, False),
+    ("poiseuille__poiseuille_bnd_pressure.json", REFCFG, 300, False),
 ]
 
 
@@ -161,11 +189,12 @@ def test_host_matches_oracle_on_reference_grid(name, folder, steps, textbook, st
     for k in ("analyticalSolution", "errorL2", "errorMax", "errorGRE", "analyticalSolutionExcludeSurface",
               "postprocessing", "poiseuillePressureGradient"):
         s.pop(k, None)
-    if name == "poiseuille__poiseuille_bnd.json":
-        # a 2e-3 density jump instead of 8e-7: at velocities of ~1e-7 the
-        # oracle's unshifted FP64 populations (~0.1) leave only ~1e-10
-        # relative precision in u; the semantics are the same
-        s["boundary"]["cube"]["+x"]["pressure"] = 1.002
+    for spec in s["boundary"].get("cube", {}).values():
+        if spec.get("pressure", 1.0) != 1.0:
+            # a 2e-3 density jump instead of 8e-7: at velocities of ~1e-7
+            # the oracle's unshifted FP64 populations (~0.1) leave only
+            # ~1e-10 relative precision in u; the semantics are the same
+            spec["pressure"] = 1.002
     h, rho, u, done = host_run(cfg, tmp_path, storage)
     assert done == steps
     r_rho, r_u = oracle_run(cfg, h, steps, textbook)
