@@ -68,9 +68,16 @@ extern "C" {
 #define LBM_FACE_WALL_NEEM 9 /* 'wall'/'neem': feq(rho, u_w) + f(n1) - feq(rho1, u1), n1 the inward
                                 neighbour (src/lbm/bnd/bnd_wall.h:187-300); at least 3 cells along the
                                 normal, no wet-node, Zou-He or NEEM face next to it */
-#define LBM_FACE_WALL_NEBB 10 /* 'wall'/'nebb', D2Q9 only: normal population reflected, diagonals
-                                 from the known ones (src/lbm/bnd/bnd_wall.h:303-479); the wall
-                                 velocity must be tangential */
+#define LBM_FACE_WALL_NEBB 10 /* 'wall'/'nebb' with 'velocity', D2Q9 only: normal population
+                                 reflected, diagonals from the known ones (apply_constV,
+                                 src/lbm/bnd/bnd_wall.h:404-472); the velocity must be tangential */
+#define LBM_FACE_WALL_NEBB0 11 /* 'wall'/'nebb' without 'velocity', D2Q9 only: the reference's
+                                  apply_0 (src/lbm/bnd/bnd_wall.h:353-400): main populations
+                                  reflected wherever a neighbour is missing, then its diagonal
+                                  formulas, which read the two diagonals that enter through the
+                                  wall; no rule writes those, so they hold their initial value
+                                  (taken as the equilibrium at rho 1, u 0) or, on a periodic-
+                                  pressure face, the value that face supplies */
 
 /* cell flags (optional byte per cell) */
 #define LBM_CELL_FLUID 0

@@ -17,7 +17,7 @@ D2Q9, D3Q19 = 9, 19
 STORE = {"f32": 0, "f64": 1, "f16": 2, "bf16": 3}
 BYTES = {"f32": 4, "f64": 8, "f16": 2, "bf16": 2}
 FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6,
-        "periodic_pressure": 7, "wall_eq": 8, "wall_neem": 9, "wall_nebb": 10}
+        "periodic_pressure": 7, "wall_eq": 8, "wall_neem": 9, "wall_nebb": 10, "wall_nebb0": 11}
 FACE_IDS = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5}
 
 

@@ -352,10 +352,11 @@ __device__ __forceinline__ void neem_correct(const G &a, int f, typename Store<S
 
 __device__ __forceinline__ bool full_cell_kind(int k) {
   return k == LBM_FACE_VELOCITY_ZH || k == LBM_FACE_PRESSURE_ZH || k == LBM_FACE_PRESSURE_NEEM ||
-         k == LBM_FACE_VELOCITY_NEEM || k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB;
+         k == LBM_FACE_VELOCITY_NEEM || k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB ||
+         k == LBM_FACE_WALL_NEBB0;
 }
 __device__ __forceinline__ bool wet_kind(int k) {
-  return k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB;
+  return k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEEM || k == LBM_FACE_WALL_NEBB || k == LBM_FACE_WALL_NEBB0;
 }
 
 // Post-streaming population j of the cell (x1, y1, z1) (index n1, link mask
@@ -433,6 +434,68 @@ __device__ __forceinline__ void wet_wall(const G &a, int x, int y, int z, int f,
   const int ax = f >> 1, s = (f & 1) ? 1 : -1;
   const int k = a.face[f].kind;
   auto cax = [](int j, int d) { return d == 0 ? L::cx[j] : (d == 1 ? L::cy[j] : L::cz[j]); };
+  if (k == LBM_FACE_WALL_NEBB0) {
+    if constexpr (Q == 9) {
+      // the two diagonals entering through the wall: their initial value
+      // (0 in shifted form at rho 1, u 0), or on a periodic-pressure face
+      // the value that face supplies (it writes every population of its
+      // cells before streaming)
+      int pg = -1;
+      if (ax != 0 && (x == 0 || x == a.nx - 1)) {
+        const int g = x == 0 ? 0 : 1;
+        if (a.face[g].kind == LBM_FACE_PERIODIC_PRESSURE) pg = g;
+      }
+      if (ax != 1 && (y == 0 || y == a.ny - 1)) {
+        const int g = y == 0 ? 2 : 3;
+        if (a.face[g].kind == LBM_FACE_PERIODIC_PRESSURE) pg = g;
+      }
+      C prc = 1, puc[3] = {0, 0, 0};
+      long long pc = 0;
+      if (pg >= 0) {
+        const int ox = (pg >> 1) == 0 ? ((pg & 1) ? 0 : a.nx - 1) : x;
+        const int oy = (pg >> 1) == 1 ? ((pg & 1) ? 0 : a.ny - 1) : y;
+        pc = (long long)ox + (long long)a.px * (long long)oy;
+        cell_rho_u<Q, ST>(src, plane, pc, a.F[0], a.F[1], a.F[2], prc, puc);
+      }
+      const C pusq = puc[0] * puc[0] + puc[1] * puc[1];
+#pragma unroll
+      for (int j = 4; j < 8; ++j) {
+        if (cax(j, ax) * s >= 0 || ((hold >> j) & 1u)) continue;
+        C v = 0;
+        if (pg >= 0) {
+          const C cu = C(L::cx[j]) * puc[0] + C(L::cy[j]) * puc[1];
+          v = C(Store<ST>::ld(src + (long long)j * plane + pc)) +
+              C(L::w[j]) * (C(a.face[pg ^ 1].rho) - prc) * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * pusq);
+        }
+        h[j] = R(v);
+      }
+      // every main population whose neighbour is missing reflected
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        if (((missing >> L::opp[d]) & 1u) && !((hold >> L::opp[d]) & 1u)) h[L::opp[d]] = h[d];
+      // the diagonal formulas of the branch the outward normal selects
+      C n[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) n[j] = C(h[j]);
+      if (ax == 0 && s < 0) {
+        n[6] = C(h[4]) + C(0.5) * (C(h[3]) - C(h[2]));
+        n[7] = C(h[5]) - C(0.5) * (C(h[3]) - C(h[2]));
+      } else if (ax == 0) {
+        n[4] = C(h[6]) - C(0.5) * (C(h[3]) - C(h[2]));
+        n[5] = C(h[7]) + C(0.5) * (C(h[3]) - C(h[2]));
+      } else if (s < 0) {
+        n[5] = C(h[7]) - C(0.5) * (C(h[1]) - C(h[0]));
+        n[6] = C(h[4]) + C(0.5) * (C(h[1]) - C(h[0]));
+      } else {
+        n[7] = C(h[5]) + C(0.5) * (C(h[1]) - C(h[0]));
+        n[4] = C(h[6]) - C(0.5) * (C(h[1]) - C(h[0]));
+      }
+#pragma unroll
+      for (int j = 4; j < 8; ++j)
+        if (!((hold >> j) & 1u)) h[j] = R(n[j]);
+    }
+    return;
+  }
   C dr = 0, ws = 0;
   int cnt = 0;
 #pragma unroll

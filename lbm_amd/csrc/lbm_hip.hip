@@ -478,8 +478,9 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
     const lbm_bc_desc &d = bcs[i];
     if (d.face < 0 || d.face >= 2 * h->dim) return fail(h, LBM_EINVAL, "lbm_set_bc: face index out of range");
     if (seen[d.face]) return fail(h, LBM_EINVAL, "lbm_set_bc: face described twice");
-    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_WALL_NEBB) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
-    if (d.kind == LBM_FACE_WALL_NEBB && h->Q != 9) return fail(h, LBM_EINVAL, "lbm_set_bc: NEBB walls are D2Q9 only");
+    if (d.kind < LBM_FACE_PERIODIC || d.kind > LBM_FACE_WALL_NEBB0) return fail(h, LBM_EINVAL, "lbm_set_bc: unknown face kind");
+    if ((d.kind == LBM_FACE_WALL_NEBB || d.kind == LBM_FACE_WALL_NEBB0) && h->Q != 9)
+      return fail(h, LBM_EINVAL, "lbm_set_bc: NEBB walls are D2Q9 only");
     if (d.kind == LBM_FACE_WALL_NEBB && d.value[d.face >> 1] != 0.0)
       return fail(h, LBM_EINVAL, "lbm_set_bc: a NEBB wall velocity must be tangential");
     seen[d.face] = true;
@@ -500,7 +501,7 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
       if ((k == LBM_FACE_PRESSURE_ABB || k == LBM_FACE_PRESSURE_NEEM || k == LBM_FACE_VELOCITY_NEEM ||
            k == LBM_FACE_WALL_NEEM) && na < 3)
         return fail(h, LBM_EINVAL, "lbm_set_bc: anti-bounce-back and NEEM faces need at least 3 cells along their normal");
-      if ((k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEBB) && na < 2)
+      if ((k == LBM_FACE_WALL_EQ || k == LBM_FACE_WALL_NEBB || k == LBM_FACE_WALL_NEBB0) && na < 2)
         return fail(h, LBM_EINVAL, "lbm_set_bc: wet-node walls need at least 2 cells along their normal");
     }
   }

@@ -154,6 +154,7 @@ class LBMBnd_wallWetnode : public LBMBndInterface {
   LBMBnd_wallWetnode(const Surface* s, const json& props, GInt dim, int kind) : m_s(s), m_kind(kind) {
     if (kind == LBM_FACE_WALL_NEBB && dim != 2) TERMM(-1, "Not implemented for this distribution!");
     if (props.count("velocity")) {
+      m_hasVelocity = true;
       auto v = required_value<std::vector<GDouble>>(props, "velocity");
       if ((GInt)v.size() < dim) TERMM(-1, "wall velocity needs dim entries");
       for (GInt d = 0; d < dim; ++d) m_value[d] = v[d];
@@ -164,6 +165,8 @@ class LBMBnd_wallWetnode : public LBMBndInterface {
     BndRule r;
     r.kind = m_kind;
     r.value = m_value;
+    // without 'velocity' the reference runs apply_0 (bnd_wall.h:353-400)
+    if (m_kind == LBM_FACE_WALL_NEBB && !m_hasVelocity) r.kind = LBM_FACE_WALL_NEBB0;
     if (m_kind == LBM_FACE_WALL_NEBB) {
       // the reference reflects the normal population with the normal
       // velocity of the cell; this build takes the wall's, so only
@@ -179,6 +182,7 @@ class LBMBnd_wallWetnode : public LBMBndInterface {
  private:
   const Surface* m_s;
   int m_kind;
+  bool m_hasVelocity = false;
   std::array<GDouble, 3> m_value{{0, 0, 0}};
 };
 

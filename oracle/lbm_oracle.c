@@ -173,7 +173,9 @@ int orc_macros(int Q, int64_t ncell, const double *fold, const double *force, do
  * Wet-node walls: lim1 / lim2 bit i = population i enters the limited
  * density with constant 1 / 2 (LBMBnd_wallWetnode, bnd_wetnode.h:20-55; the
  * rest population is a lim1 bit); flags bit 0: the wall has a velocity
- * (apply_constV), bit 1: empty limited set (a corner, density unchanged);
+ * (apply_constV), bit 1: empty limited set (a corner, density unchanged),
+ * bit 2: leave vars untouched (the reference writes the wall's velocity and
+ * density there, which later records of the same step may read);
  * axis/sign: the outward normal. */
 typedef struct {
   int64_t cell;
@@ -345,7 +347,12 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
       /* DirichletEQ::apply: the wall velocity into vars, the limited
        * density, then every population = feq(rho, u_w) (w_i rho without a
        * velocity, VALZERO) */
+      double vloc[4];
       double *v = vars + c * nv;
+      if (r->flags & 4) {
+        for (int k = 0; k < nv; ++k) vloc[k] = v[k];
+        v = vloc;
+      }
       for (int d = 0; d < L.dim; ++d) v[d] = r->u[d];
       wet_density(&L, r, fold + c * Q, v);
       const double rho = v[L.dim];
@@ -382,7 +389,12 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
        * -/+ 2/3 rho u_n; the two unknown diagonals from the known ones,
        * the tangential difference and the wall velocity (Zou & He 1997). */
       if (Q != 9) return -3;
+      double vloc[4];
       double *v = vars + c * nv;
+      if (r->flags & 4) {
+        for (int k = 0; k < nv; ++k) vloc[k] = v[k];
+        v = vloc;
+      }
       double *f = fold + c * Q;
       wet_density(&L, r, f, v);
       for (int d = 0; d < L.dim; ++d) v[d] = r->u[d];
@@ -406,6 +418,38 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
       } else {
         f[7] = f[5] + 0.5 * (f[1] - f[0]) - 0.5 * rho * uw[0] + 1.0 / 6.0 * rho * uw[1];
         f[4] = f[6] - 0.5 * (f[1] - f[0]) + 0.5 * rho * uw[0] + 1.0 / 6.0 * rho * uw[1];
+      }
+    } else if (r->kind == 11) {
+      /* NEBB without a wall velocity, LBMBnd_wallNEBB::apply_0
+       * (bnd_wall.h:353-400): zero velocity, limited density, every main
+       * population whose neighbour is missing reflected, then the
+       * diagonal formulas of the branch its outward normal selects; the
+       * populations they read hold whatever fold holds (values no rule has
+       * written this step keep older ones). */
+      if (Q != 9) return -3;
+      double vloc[4];
+      double *v = vars + c * nv;
+      if (r->flags & 4) {
+        for (int k = 0; k < nv; ++k) vloc[k] = v[k];
+        v = vloc;
+      }
+      double *f = fold + c * Q;
+      for (int d = 0; d < L.dim; ++d) v[d] = 0.0;
+      wet_density(&L, r, f, v);
+      for (int d = 0; d < 4; ++d)
+        if (nbr[c * (Q - 1) + d] < 0) f[L.opp[d]] = f[d];
+      if (r->axis == 0 && r->sign < 0) {
+        f[6] = f[4] + 0.5 * (f[3] - f[2]);
+        f[7] = f[5] - 0.5 * (f[3] - f[2]);
+      } else if (r->axis == 0) {
+        f[4] = f[6] - 0.5 * (f[3] - f[2]);
+        f[5] = f[7] + 0.5 * (f[3] - f[2]);
+      } else if (r->sign < 0) {
+        f[5] = f[7] - 0.5 * (f[1] - f[0]);
+        f[6] = f[4] + 0.5 * (f[1] - f[0]);
+      } else {
+        f[7] = f[5] + 0.5 * (f[1] - f[0]);
+        f[4] = f[6] - 0.5 * (f[1] - f[0]);
       }
     } else if (r->kind == 5 || r->kind == 6) {
       /* NEEM faces (Guo, Zheng & Shi, Chin. Phys. 11 366 (2002)): every

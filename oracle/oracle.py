@@ -24,10 +24,10 @@ LIB = os.path.join(HERE, "liblbm_oracle.so")
 REF_HILBERT = os.path.join(HERE, "_ref", "libref_hilbert.so")
 
 FACE = {"periodic": 0, "wall": 1, "pressure_abb": 2, "velocity_zh": 3, "pressure_zh": 4, "pressure_neem": 5, "velocity_neem": 6,
-        "periodic_pressure": 7, "wall_eq": 8, "wall_neem": 9, "wall_nebb": 10}
+        "periodic_pressure": 7, "wall_eq": 8, "wall_neem": 9, "wall_nebb": 10, "wall_nebb0": 11}
 # faces whose rule replaces the populations of the whole face cell
-FULL = (3, 4, 5, 6, 8, 9, 10)
-WET = (8, 9, 10)
+FULL = (3, 4, 5, 6, 8, 9, 10, 11)
+WET = (8, 9, 10, 11)
 EPS = 2.220446049250313e-16
 
 
@@ -206,9 +206,26 @@ class Problem:
         wet = []
         after = []
         zh = []
+        pre = []
         PP = FACE["periodic_pressure"]
         for cidx in np.nonzero(self.fluid & (nbr < 0).any(axis=1))[0]:
             co = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
+            # a periodic-pressure face writes every population of its cells
+            # that does not stream in, before any other rule (preApply,
+            # bnd_periodic.h:102-108); the rules of other faces overwrite
+            for f in range(2 * dim):
+                ax, side = f // 2, f % 2
+                if self.face_kind[f] == PP and co[ax] == (ext[ax] - 1 if side else 0):
+                    r = OrcBC()
+                    r.cell = int(cidx)
+                    r.kind = 7
+                    r.rho = self.face_rho[f ^ 1]
+                    c1 = list(co)
+                    c1[ax] = 0 if side == 1 else ext[ax] - 1
+                    r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
+                    r.dirmask = int(sum(1 << i for i in range(Q - 1) if nbr[cidx, i] < 0))
+                    pre.append(r)
+                    break
             # the whole-cell rule: the full-cell face of highest priority the
             # cell lies on (ties to the lower face index)
             F = None
@@ -254,12 +271,7 @@ class Problem:
                     r.n2 = c2[0] + nx * (c2[1] + ny * c2[2])
                     dest.append(r)
                 elif kind == FACE["periodic_pressure"]:
-                    r.kind = 7
-                    r.rho = self.face_rho[key ^ 1]  # pressure of the other face
-                    c1 = [int(x[cidx]), int(y[cidx]), int(z[cidx])]
-                    c1[ax] = 0 if side == 1 else ext[ax] - 1
-                    r.n1 = c1[0] + nx * (c1[1] + ny * c1[2])
-                    dest.append(r)
+                    pass  # written by the face's record above
                 elif kind in (FACE["pressure_neem"], FACE["velocity_neem"]):
                     # the face record (below) replaces every population; with
                     # a solid inward neighbour the missing ones bounce back
@@ -293,7 +305,9 @@ class Problem:
                 # tangential velocity, so no division (the reference divides
                 # by the cell's own normal velocity there, zero once the wall
                 # has acted)
-                r.flags = (0 if r.kind == 10 else 1) | (2 if empty else 0)
+                # bit 2: the device's other rules read the state before
+                # the step, never what a wall wrote into vars
+                r.flags = (0 if r.kind in (10, 11) else 1) | (2 if empty else 0) | 4
                 r.u = (ct.c_double * 3)(*self.face_u[F])
                 c1 = list(co)
                 c1[ax] += -1 if side else 1
@@ -319,7 +333,7 @@ class Problem:
                 r.rho = self.face_rho[f]
                 zh.append(r)
                 break
-        allr = recs + wet + after + zh
+        allr = pre + recs + wet + after + zh
         self.nrec = len(allr)
         self.recs = (OrcBC * max(1, len(allr)))(*allr)
 
@@ -512,6 +526,8 @@ class GridProblem(Problem):
                         c, nrm3, self.Q, lambda i: per or nbr_table[cidx, opp[i]] >= 0,
                         lambda i: nbr_table[cidx, i] < 0 and not per)
                     r.kind = {"equilibrium": 8, "neem": 9, "nebb": 10}[spec["model"]]
+                    if r.kind == 10 and "velocity" not in spec:
+                        r.kind = 11  # LBMBnd_wallNEBB::apply_0
                     a = d // 2
                     r.axis = a
                     r.sign = 1 if d % 2 else -1

@@ -238,3 +238,79 @@ def test_reference_couette_wetnode_thresholds(name):
     print(f"{name}: steps {steps} errorMax {emax:.6e} (< {s['errorMax']}) errorL2 {l2:.6e} (< {s['errorL2']})")
     assert emax < max(s["errorMax"], 1e-14)
     assert l2 < max(s["errorL2"], 1e-14)
+
+
+def test_nebb_without_velocity_is_no_slip():
+    """The reference's no-velocity NEBB (apply_0, bnd_wall.h:353-400) with
+    the two diagonals that enter through the wall at their initial value w:
+    after every step the wall cells carry zero velocity exactly (its
+    formulas make the x and y momentum cancel), and the populations that
+    streamed in from above are kept."""
+    faces = {"-y": ("wall_nebb0", None, 1.0, 4), "+y": ("wall_nebb0", None, 1.0, 2)}
+    shape = (10, 8)
+    N = 80
+    rng = np.random.default_rng(9)
+    rho0 = 1 + 0.02 * rng.standard_normal(N)
+    u0 = 0.03 * rng.standard_normal((N, 2))
+    wall = (np.arange(N) // 10 == 0) | (np.arange(N) // 10 == 7)
+    rho0[wall] = 1.0
+    u0[wall] = 0.0
+    p = O.Problem(shape, faces=faces, omega=1 / 0.7)
+    p.init(rho0, u0)
+    c, w, _ = O.tables(9)
+    for _ in range(5):
+        p.step(3)
+        rho, u = p.macros()
+        assert np.abs(u[wall]).max() < 1e-16
+        f1 = p.fold.reshape(N, 9)
+        # the entering diagonals keep their initial value w
+        for x in range(10):
+            assert f1[x, 4] == w[4] and f1[x, 7] == w[7]
+            assert f1[x + 70, 5] == w[5] and f1[x + 70, 6] == w[6]
+
+
+@pytest.mark.parametrize("name", ["poiseuille__poiseuille_bnd_eq.json", "poiseuille__poiseuille_bnd_NEEM.json",
+                                  "poiseuille__poiseuille_bnd_NEBB.json", "poiseuille__poiseuille_bnd_pressure.json",
+                                  "poiseuille__poiseuille_bnd_pressure_neem2.json"])
+def test_reference_poiseuille_wetnode_thresholds(name):
+    """The reference's pressure-driven Poiseuille cases with wet-node walls
+    (aligned grids, periodic pressure pair or anti-bounce-back faces),
+    stepped by the oracle on the reference grid in the reference's order
+    until their own convergence criterion, meet their own errorMax against
+    poiseuille2D_1 (src/lbm/analytical_solutions.h:51-59) over the cells
+    outside the excluded surfaces.  The errors land just under the
+    thresholds (e.g. 7.952e-6 < 7.96e-6, 2.739e-6 < 2.74e-6), as the values
+    of the runs that set them would."""
+    from tests.test_grid import host_grid
+
+    cfg = _load(name)
+    s = cfg["solver"]
+    bnd = s["boundary"]
+    order = []
+    for g in sorted(bnd):
+        for f in sorted(bnd[g]):
+            order.append((f"{g}_{f}" if len(bnd[g]) > 1 else g, bnd[g][f]))
+    h = host_grid(cfg)
+    nx, ny, _ = h["nxyz"]
+    sidx = h["sidx"]
+    coords = np.stack([sidx % nx, (sidx // nx) % ny, sidx // (nx * ny)], 1)
+    nu = s["ma"] / s["reynoldsnumber"]
+    omega = 2.0 / (1.0 + 2.0 * nu * 2.0 ** cfg["partitionLevel"])  # src/lbm/solver.cpp:118-119
+    p = O.GridProblem(2, h["nb"], h["surf"], order, omega, coords=coords)
+    p.init_reference()
+    prev = None
+    while p.steps < s["maxSteps"]:
+        p.step(100)
+        _, u = p.macros()
+        if prev is not None and np.abs(u - prev).sum() / 100 < s["convergence"]:
+            break
+        prev = u
+    a = s["poiseuillePressureGradient"] / (2 * nu) * h["cen"][:, 1] * (1.0 - h["cen"][:, 1])
+    excl = set()
+    for sn in s["analyticalSolutionExcludeSurface"]:
+        excl |= {int(c) for c in h["surf"][sn][0]}
+    keep = np.array([i not in excl for i in range(h["n"])])
+    d = np.sqrt((u[:, 0] - a) ** 2 + u[:, 1] ** 2)[keep]
+    print(f"{name}: steps {p.steps} errorMax {d.max():.6e} (< {s['errorMax']})")
+    assert np.isfinite(d).all() and d.max() < s["errorMax"]
+
