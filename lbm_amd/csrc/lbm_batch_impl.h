@@ -72,7 +72,10 @@ struct lbm_batch {
   DomainDev *doms_d = nullptr;
   int4 *tiles_d = nullptr;
   long long ntiles = 0;
+  int *ids_d = nullptr;  // interior tile indices, then the others
+  int n_int = 0, n_bnd = 0;
   int tx = 64, blocks = 0;  // blocks == 0: one block per tile
+  int xcd_chunks = 1;       // LBM_BATCH_XCD=0 turns the per-XCD tile ranges off
   int *active_d = nullptr;
   double *res_d = nullptr, *mass_d = nullptr, *partials = nullptr;
   long long *conv_d = nullptr, *cycle_d = nullptr;
@@ -102,6 +105,9 @@ static int bfail(lbm_batch *b, int code, const std::string &msg) {
 static int batch_build_tiles(lbm_batch *b, int tx) {
   const int ty = 256 / tx;
   std::vector<int4> tiles;
+  // interior tiles: full, no face, wrap or solid within one cell of them;
+  // they run in a kernel compiled without the boundary paths
+  std::vector<int> ids_int, ids_bnd;
   for (int d = 0; d < b->ndom; ++d) {
     DomainDev &D = b->doms[d];
     D.tile_begin = (int)tiles.size();
@@ -122,19 +128,33 @@ static int batch_build_tiles(lbm_batch *b, int tx) {
                     need = 1;
                     break;
                   }
-            // periodic wrap reaches the opposite edge
-            if (!need && (D.per[0] || D.per[1] || D.per[2])) need = 1;
           }
+          const bool interior = !need && x0 >= 1 && x0 + tx <= D.nx - 1 && y0 >= 1 && y0 + ty <= D.ny - 1 &&
+                                (D.nz == 1 || (z >= 1 && z <= D.nz - 2));
+          // periodic wrap reaches the opposite edge
+          if (!sol.empty() && !need && (D.per[0] || D.per[1] || D.per[2])) need = 1;
+          (interior ? ids_int : ids_bnd).push_back((int)tiles.size());
           tiles.push_back(make_int4(d, x0, y0, z | (need << 30)));
         }
     D.tile_end = (int)tiles.size();
   }
-  if (tiles.empty()) tiles.push_back(make_int4(0, 0, 0, 0));
+  if (tiles.empty()) {
+    ids_bnd.push_back(0);
+    tiles.push_back(make_int4(0, 0, 0, 0));
+  }
   if (b->tiles_d) BHIPCHK(b, hipFree(b->tiles_d));
   b->tiles_d = nullptr;
   b->ntiles = (long long)tiles.size();
   BHIPCHK(b, hipMalloc(&b->tiles_d, tiles.size() * sizeof(int4)));
   BHIPCHK(b, hipMemcpy(b->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  std::vector<int> ids(ids_int);
+  ids.insert(ids.end(), ids_bnd.begin(), ids_bnd.end());
+  if (b->ids_d) BHIPCHK(b, hipFree(b->ids_d));
+  b->ids_d = nullptr;
+  BHIPCHK(b, hipMalloc(&b->ids_d, ids.size() * sizeof(int)));
+  BHIPCHK(b, hipMemcpy(b->ids_d, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+  b->n_int = (int)ids_int.size();
+  b->n_bnd = (int)ids_bnd.size();
   BHIPCHK(b, hipMemcpy(b->doms_d, b->doms.data(), b->doms.size() * sizeof(DomainDev), hipMemcpyHostToDevice));
   if (b->partials) BHIPCHK(b, hipFree(b->partials));
   b->partials = nullptr;
@@ -157,6 +177,7 @@ static BatchArgs batch_args(lbm_batch *b, int parity, int mode) {
   a.tiles = b->tiles_d;
   a.ntiles = (int)b->ntiles;
   a.tx = b->tx;
+  a.xcd_chunks = b->xcd_chunks;
   a.active = b->active_d;
   if (mode >= 1) {
     // snapshot written by the step before the check, read by the check
@@ -167,11 +188,26 @@ static BatchArgs batch_args(lbm_batch *b, int parity, int mode) {
   return a;
 }
 
-static int batch_grid(const lbm_batch *b) { return b->blocks > 0 ? b->blocks : (int)b->ntiles; }
+// blocks of one of the two launches: one per tile, or the persistent grid
+// shared in proportion to the tiles (at least one block per XCD)
+static int batch_grid(const lbm_batch *b, int n) {
+  if (b->blocks <= 0) return n;
+  const long long g = (long long)b->blocks * n / std::max<long long>(1, b->ntiles);
+  return (int)std::max<long long>(std::min<long long>(8, n), g);
+}
 
 static void batch_launch(lbm_batch *b, int parity, int mode) {
-  const BatchArgs a = batch_args(b, parity, mode);
-  dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b), b->stream, mode);
+  BatchArgs a = batch_args(b, parity, mode);
+  if (b->n_int > 0) {
+    a.ids = b->ids_d;
+    a.nids = b->n_int;
+    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_int), b->stream, mode, true);
+  }
+  if (b->n_bnd > 0) {
+    a.ids = b->ids_d + b->n_int;
+    a.nids = b->n_bnd;
+    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_bnd), b->stream, mode, false);
+  }
 }
 
 extern "C" {
@@ -183,6 +219,7 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
   const int es = esize_of(storage);
   if (!es) return bfail(nullptr, LBM_EINVAL, "lbm_batch_create: unknown storage");
   lbm_batch *b = new lbm_batch();
+  if (const char *e = getenv("LBM_BATCH_XCD")) b->xcd_chunks = atoi(e) != 0;
   b->Q = lattice;
   b->dim = lattice == LBM_D2Q9 ? 2 : 3;
   b->storage = storage;
@@ -527,7 +564,7 @@ int lbm_batch_destroy(lbm_batch *b) {
     if (b->buf[k]) (void)hipFree(b->buf[k]);
     if (b->snap[k]) (void)hipFree(b->snap[k]);
   }
-  void *ptrs[] = {b->flags, b->links, b->doms_d, b->tiles_d, b->active_d, b->res_d, b->mass_d, b->partials, b->conv_d, b->cycle_d};
+  void *ptrs[] = {b->flags, b->links, b->doms_d, b->tiles_d, b->ids_d, b->active_d, b->res_d, b->mass_d, b->partials, b->conv_d, b->cycle_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -627,7 +627,12 @@ __device__ __forceinline__ void wet_wall(const G &a, int x, int y, int z, int f,
 // nz, px, per[3], face[6], omega, F[3], snap_in, snap_out (the kernel
 // arguments for one domain, or a domain of a batch).  red must be zeroed by
 // the caller; MODE >= 2 accumulates this cell's sums into it.
-template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, typename G>
+// TOPO 0: any domain.  The boundary paths are compiled out when the caller
+// guarantees that no population of the cell crosses a face or comes from a
+// solid: TOPO 1, every neighbour is a fluid cell inside the domain (no
+// wrap); TOPO 2, every axis is periodic and there are no solids (wraps
+// only).
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO = 0, typename G>
 __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, double (&red)[NSUM]) {
   using L = Lat<Q>;
   using C = typename Store<ST>::C;
@@ -653,13 +658,24 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
     // neighbour coordinates per axis with periodic wrap; v* = exists
     int xm = x - 1, xp = x + 1, ym = y - 1, yp = y + 1, zm = z - 1, zp = z + 1;
     bool vxm = true, vxp = true, vym = true, vyp = true, vzm = true, vzp = true;
-    if (x == 0) { if (a.per[0]) xm = a.nx - 1; else vxm = false; }
-    if (x == a.nx - 1) { if (a.per[0]) xp = 0; else vxp = false; }
-    if (y == 0) { if (a.per[1]) ym = a.ny - 1; else vym = false; }
-    if (y == a.ny - 1) { if (a.per[1]) yp = 0; else vyp = false; }
-    if (Q == 19) {
-      if (z == 0) { if (a.per[2]) zm = a.nz - 1; else vzm = false; }
-      if (z == a.nz - 1) { if (a.per[2]) zp = 0; else vzp = false; }
+    if (TOPO == 2) {
+      if (x == 0) xm = a.nx - 1;
+      if (x == a.nx - 1) xp = 0;
+      if (y == 0) ym = a.ny - 1;
+      if (y == a.ny - 1) yp = 0;
+      if (Q == 19) {
+        if (z == 0) zm = a.nz - 1;
+        if (z == a.nz - 1) zp = 0;
+      }
+    } else if (TOPO == 0) {
+      if (x == 0) { if (a.per[0]) xm = a.nx - 1; else vxm = false; }
+      if (x == a.nx - 1) { if (a.per[0]) xp = 0; else vxp = false; }
+      if (y == 0) { if (a.per[1]) ym = a.ny - 1; else vym = false; }
+      if (y == a.ny - 1) { if (a.per[1]) yp = 0; else vyp = false; }
+      if (Q == 19) {
+        if (z == 0) { if (a.per[2]) zm = a.nz - 1; else vzm = false; }
+        if (z == a.nz - 1) { if (a.per[2]) zp = 0; else vzp = false; }
+      }
     }
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
@@ -816,8 +832,8 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
   // checked by its cell index; where it is not in the wave the face cell
   // gathers (neem_face).
   bool from_lane = false;
-  if (a.face[0].kind == LBM_FACE_PRESSURE_NEEM || a.face[0].kind == LBM_FACE_VELOCITY_NEEM ||
-      a.face[1].kind == LBM_FACE_PRESSURE_NEEM || a.face[1].kind == LBM_FACE_VELOCITY_NEEM) {
+  if (TOPO == 0 && (a.face[0].kind == LBM_FACE_PRESSURE_NEEM || a.face[0].kind == LBM_FACE_VELOCITY_NEEM ||
+                    a.face[1].kind == LBM_FACE_PRESSURE_NEEM || a.face[1].kind == LBM_FACE_VELOCITY_NEEM)) {
     const bool lo = x == 0 && (a.face[0].kind == LBM_FACE_PRESSURE_NEEM || a.face[0].kind == LBM_FACE_VELOCITY_NEEM);
     const bool hi = x == a.nx - 1 &&
                     (a.face[1].kind == LBM_FACE_PRESSURE_NEEM || a.face[1].kind == LBM_FACE_VELOCITY_NEEM);

@@ -579,7 +579,9 @@ static int launch_step(lbm_handle *h, int mode) {
   }
   dim3 g, b;
   launch_dims(h, g, b);
-  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0, h->w64);
+  // a periodic box without solids runs a kernel without boundary paths
+  const bool periodic_box = !h->has_flags && h->per[0] && h->per[1] && (h->dim == 2 || h->per[2]);
+  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0, h->w64, periodic_box);
   HIPCHK(h, hipGetLastError());
   if (mode == 1 || mode == 3) h->snap_last ^= 1;
   h->cur ^= 1;

@@ -44,6 +44,9 @@ struct BatchArgs {
   const int4 *tiles;
   int ntiles;
   int tx;  // tile width in cells (tile = tx x (256/tx))
+  int xcd_chunks;  // 1: each XCD walks a contiguous range of tiles
+  const int *ids;  // the tiles this launch covers (indices into tiles)
+  int nids;
   const int *active;
   void *snap_in;
   void *snap_out;
@@ -54,13 +57,15 @@ template <int Q, typename ST>
 struct StepL {
   // mode: 0 plain, 1 snapshot, 2 check, 3 check + snapshot; nt: nontemporal
   // loads/stores of the plain step; w64: 64-bit population offsets
-  static void run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64);
+  static void run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64,
+                  bool periodic_box);
 };
 
 template <int Q, typename ST>
 struct BatchStepL {
-  // mode: 0 plain, 1 snapshot, 2 check
-  static void run(const BatchArgs &a, int grid, hipStream_t s, int mode);
+  // mode: 0 plain, 1 snapshot, 2 check; interior: the tiles in a.ids have
+  // no boundary within one cell (compiled without the boundary paths)
+  static void run(const BatchArgs &a, int grid, hipStream_t s, int mode, bool interior);
 };
 
 }  // namespace lbm

@@ -9,24 +9,33 @@ namespace lbm {
 // storage (FP64 arithmetic) compiles to 129-131 VGPRs unconstrained, i.e. 3
 // waves; held to 128 it keeps 4 waves without spilling (32-bit offsets) and
 // runs 3-4 % faster.  FP64 storage spills under that limit and is left free.
-template <int Q, typename ST>
+// A fully periodic box without solids (TOPO 2) has no boundary code and
+// needs 74 VGPRs for D3Q19 FP32, i.e. 6 waves, unconstrained.
+template <int Q, typename ST, int TOPO>
 struct StepWaves {
   static constexpr int value = 1;
 };
 template <>
-struct StepWaves<19, float> {
+struct StepWaves<19, float, 0> {
   static constexpr int value = 4;
 };
+#ifndef LBM_BOX_WAVES
+#define LBM_BOX_WAVES 1
+#endif
+template <>
+struct StepWaves<19, float, 2> {
+  static constexpr int value = LBM_BOX_WAVES;
+};
 
-template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<Q, ST>::value))) k_step(const KArgs a) {
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<Q, ST, TOPO>::value))) k_step(const KArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   double red[NSUM];
 #pragma unroll
   for (int k = 0; k < NSUM; ++k) red[k] = 0;
-  cell_update<Q, ST, FLAGS, MODE, NT, W64>(a, x, y, z, red);
+  cell_update<Q, ST, FLAGS, MODE, NT, W64, TOPO>(a, x, y, z, red);
   if (MODE >= 2) {
     const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
     const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
@@ -34,31 +43,63 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWa
   }
 }
 
-template <int Q, typename ST, bool FLAGS, bool W64>
+template <int Q, typename ST, bool FLAGS, bool W64, int TOPO>
 static void launch_step_kernel(const KArgs &a, dim3 g, dim3 b, hipStream_t s, int mode, bool nt) {
   if (mode == 0) {
-    if (nt) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, true, W64>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, false, W64>), g, b, 0, s, a);
-  } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 1, false, W64>), g, b, 0, s, a);
-  else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 2, false, W64>), g, b, 0, s, a);
-  else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 3, false, W64>), g, b, 0, s, a);
+    if (nt) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, true, W64, TOPO>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 0, false, W64, TOPO>), g, b, 0, s, a);
+  } else if (mode == 1) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 1, false, W64, TOPO>), g, b, 0, s, a);
+  else if (mode == 2) hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 2, false, W64, TOPO>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_step<Q, ST, FLAGS, 3, false, W64, TOPO>), g, b, 0, s, a);
 }
 template <int Q, typename ST>
-void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64) {
+void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64,
+                       bool periodic_box) {
   if (w64) {
-    if (flags) launch_step_kernel<Q, ST, true, true>(a, g, b, s, mode, nt);
-    else launch_step_kernel<Q, ST, false, true>(a, g, b, s, mode, nt);
+    if (flags) launch_step_kernel<Q, ST, true, true, 0>(a, g, b, s, mode, nt);
+    else if (periodic_box) launch_step_kernel<Q, ST, false, true, 2>(a, g, b, s, mode, nt);
+    else launch_step_kernel<Q, ST, false, true, 0>(a, g, b, s, mode, nt);
   } else {
-    if (flags) launch_step_kernel<Q, ST, true, false>(a, g, b, s, mode, nt);
-    else launch_step_kernel<Q, ST, false, false>(a, g, b, s, mode, nt);
+    if (flags) launch_step_kernel<Q, ST, true, false, 0>(a, g, b, s, mode, nt);
+    else if (periodic_box) launch_step_kernel<Q, ST, false, false, 2>(a, g, b, s, mode, nt);
+    else launch_step_kernel<Q, ST, false, false, 0>(a, g, b, s, mode, nt);
   }
 }
 
-template <int Q, typename ST, int MODE, bool NT>
-__global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
+// Occupancy target of the batched step (waves per SIMD).  The tiles of a
+// batch are small, so the bytes in flight per CU come from occupancy: the
+// D2Q9 FP32 boundary-tile kernel (87 VGPRs, 5 waves unconstrained) runs
+// 10 % faster held to 8 waves despite the spill this costs in its boundary
+// paths.  Interior tiles have their own kernel without boundary code.
+template <int Q, typename ST, bool INT>
+struct BatchWaves {
+  static constexpr int value = 1;
+};
+template <>
+struct BatchWaves<9, float, false> {
+  static constexpr int value = 8;
+};
+
+template <int Q, typename ST, int MODE, bool NT, bool INT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchWaves<Q, ST, INT>::value))) k_batch_step(const BatchArgs a) {
   using C = typename Store<ST>::C;
   using V4 = typename Vec4<C>::T;
-  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+  // Workgroups go to the 8 XCDs in turn, and each XCD has its own L2.  With
+  // xcd_chunks the blocks of one XCD walk one contiguous range of the tile
+  // list (domain-major, rows in order), so the tiles that share cache lines
+  // (the x and y neighbours, through the shifted loads) run through the
+  // same L2 instead of fetching those lines once per XCD.
+  constexpr int NXCD = 8;
+  int k = blockIdx.x, k_end = a.nids, k_step = gridDim.x;
+  if (a.xcd_chunks && gridDim.x >= NXCD) {  // every XCD then has a block
+    const int xcd = (int)(blockIdx.x % NXCD);
+    const int chunk = (a.nids + NXCD - 1) / NXCD;
+    k = xcd * chunk + (int)(blockIdx.x / NXCD);
+    k_end = min(a.nids, (xcd + 1) * chunk);
+    k_step = ((int)gridDim.x - xcd + NXCD - 1) / NXCD;  // blocks on this XCD
+  }
+  for (; k < k_end; k += k_step) {
+    const int t = a.ids[k];
     const int4 tl = a.tiles[t];
     if (!a.active[tl.x]) continue;  // uniform per block
     const DomainDev &d = a.doms[tl.x];
@@ -86,7 +127,8 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
 #pragma unroll
     for (int k = 0; k < NSUM; ++k) red[k] = 0;
     // tiles with no solid cell within one cell of them skip the flag loads
-    if (tl.w >> 30) cell_update<Q, ST, true, MODE, NT, false>(v, x, y, z, red);
+    if (INT) cell_update<Q, ST, false, MODE, NT, false, 1>(v, x, y, z, red);
+    else if (tl.w >> 30) cell_update<Q, ST, true, MODE, NT, false>(v, x, y, z, red);
     else cell_update<Q, ST, false, MODE, NT, false>(v, x, y, z, red);
     if (MODE >= 2) {
       block_reduce_store<256>(red, a.partials, t, a.ntiles);
@@ -95,11 +137,16 @@ __global__ void __launch_bounds__(256) k_batch_step(const BatchArgs a) {
   }
 }
 
+template <int Q, typename ST, bool INT>
+static void launch_batch_kernel(const BatchArgs &a, int grid, hipStream_t s, int mode) {
+  if (mode == 0) hipLaunchKernelGGL((k_batch_step<Q, ST, 0, true, INT>), dim3(grid), dim3(256), 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL((k_batch_step<Q, ST, 1, false, INT>), dim3(grid), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_batch_step<Q, ST, 2, false, INT>), dim3(grid), dim3(256), 0, s, a);
+}
 template <int Q, typename ST>
-void BatchStepL<Q, ST>::run(const BatchArgs &a, int grid, hipStream_t s, int mode) {
-  if (mode == 0) hipLaunchKernelGGL((k_batch_step<Q, ST, 0, true>), dim3(grid), dim3(256), 0, s, a);
-  else if (mode == 1) hipLaunchKernelGGL((k_batch_step<Q, ST, 1, false>), dim3(grid), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_batch_step<Q, ST, 2, false>), dim3(grid), dim3(256), 0, s, a);
+void BatchStepL<Q, ST>::run(const BatchArgs &a, int grid, hipStream_t s, int mode, bool interior) {
+  if (interior) launch_batch_kernel<Q, ST, true>(a, grid, s, mode);
+  else launch_batch_kernel<Q, ST, false>(a, grid, s, mode);
 }
 
 }  // namespace lbm
