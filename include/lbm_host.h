@@ -48,6 +48,20 @@ void lbm_host_grid_destroy(lbm_host_grid *g);
 int lbm_host_run_file(const char *config_file, const char *storage, int64_t n, double *rho, double *u,
                       int64_t *steps_done);
 
+/* Write <file_base>.vtp, the reference's base64 binary point file
+ * (VTK::BINARY::writePoints, src/common/IO.h:414-507, as LBMSolver::output
+ * calls it, src/lbm/solver.cpp:335-381): n points with dim coordinates
+ * coords[n][dim] and nfields Float64 fields names[k] / values[k][n].  Values
+ * are written as given (the solver rounds them to 15 decimals first, as the
+ * reference's text round trip does).  Returns 0, or -1 if n < 1 or the file
+ * cannot be written. */
+int lbm_host_write_points_vtk(const char *file_base, int32_t dim, int64_t n, const double *coords, int32_t nfields,
+                              const char *const *names, const double *const *values);
+
+/* value -> "%.15f" text -> double, the reference's field round trip
+ * (toStringVector, include/common/util/string_helper.h:95-105) */
+double lbm_host_text_round_trip(double v);
+
 const char *lbm_host_last_error(void);
 
 #ifdef __cplusplus

@@ -3,6 +3,7 @@
 
 #include "../../include/lbm_host.h"
 #include "solver.h"
+#include "vtk.h"
 
 using namespace lbmh;
 
@@ -112,6 +113,23 @@ int lbm_host_run_file(const char* config_file, const char* storage, int64_t n, d
   }
   return (int)rc;
 }
+
+int lbm_host_write_points_vtk(const char* file_base, int32_t dim, int64_t n, const double* coords, int32_t nfields,
+                              const char* const* names, const double* const* values) {
+  if (!file_base || dim < 1 || dim > 3 || n < 1 || !coords || nfields < 0 || (nfields > 0 && (!names || !values))) {
+    g_host_err = "lbm_host_write_points_vtk: invalid argument";
+    return -1;
+  }
+  std::vector<vtk::Field> fields;
+  for (int32_t k = 0; k < nfields; ++k) fields.push_back({names[k], std::vector<double>(values[k], values[k] + n)});
+  if (!vtk::writePoints(file_base, dim, n, coords, fields)) {
+    g_host_err = std::string("lbm_host_write_points_vtk: cannot write ") + file_base + ".vtp";
+    return -1;
+  }
+  return 0;
+}
+
+double lbm_host_text_round_trip(double v) { return vtk::textRoundTrip(v); }
 
 const char* lbm_host_last_error(void) { return g_host_err.c_str(); }
 

@@ -2,6 +2,8 @@
 
 #include <sys/stat.h>
 
+#include "vtk.h"
+
 #include <cstring>
 
 namespace lbmh {
@@ -102,7 +104,25 @@ void LBMSolver::loadConfiguration() {
   if (m_outputDir.empty() || m_outputDir.back() != '/') m_outputDir += '/';
   m_solutionInterval = c.opt_config_value<GInt>("solution_interval", 100);
   m_solutionFile = c.opt_config_value<std::string>("solution_filename", "solution");
-  c.getObject("cellFilter");
+  // output filter (CellFilterManager, src/cell_filter.h:64-103), default
+  // {"cellFilter": "leafCells"} (src/lbm/solver.cpp:86-87); the grids of
+  // this build have one level, so a filter passes every cell or none
+  {
+    json fc = c.getObject("cellFilter");
+    if (fc.empty()) fc = json{{"cellFilter", "leafCells"}};
+    if (!fc.count("cellFilter")) TERMM(-1, "Invalid output configuration missing \"cellFilter\" key");
+    if (fc["cellFilter"].is_array()) TERMM(-1, "untested");
+    const std::string f = fc["cellFilter"].get<std::string>();
+    m_outputAll = true;
+    if (f == "targetLvl") {
+      if (!fc.count("outputLvl")) TERMM(-1, "The required configuration value is missing: outputLvl");
+      const GInt lvl = fc["outputLvl"].get<GInt>();
+      if (lvl < m_grid->partitionLvl()) TERMM(-1, "Outputting a lvl below the partition lvl is not possible!");
+      m_outputAll = lvl == m_grid->partitionLvl();
+    } else if (f != "highestLvl" && f != "lowestLvl" && f != "partitionLvl" && f != "leafCells") {
+      TERMM(-1, "Unknown output filter " + f);
+    }
+  }
   c.getObject("postprocessing");
   c.opt_config_value<bool>("assumeAxisAligned", true);
   m_refLength = c.opt_config_value<GDouble>("refLength", 1.0);
@@ -257,17 +277,15 @@ auto LBMSolver::run() -> GInt {
         haveSums = true;
       }
     }
+    // also at the last step or once converged (src/lbm/solver.cpp:194-195)
+    output(m_timeStep == m_maxSteps - 1 || converged, "");
   }
   check(lbm_synchronize(m_h), "lbm_synchronize");
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
             << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
   fetchMacros();
-  if (m_diverged) {
-    writeSolution("bdiv");
-    TERMM(-1, "Solution diverged");
-  }
-  writeSolution("");
+  if (m_diverged) TERMM(-1, "Solution diverged");
   if (m_solverCfg->has_config_value("analyticalSolution")) compareToAnalyticalResult();
   for (const auto& k : m_solverCfg->unusedConfigValues()) std::cerr << "WARNING: unused configuration value " << k << std::endl;
   std::cerr << "LBM Solver finished <||" << std::endl;
@@ -329,24 +347,41 @@ void LBMSolver::compareToAnalyticalResult() {
   if (failed) TERMM(-1, "Analytical testcase failed");
 }
 
-// Final macroscopic fields in reference cell-id order as CSV (the reference's
-// VTK point writer is not part of this build).
-void LBMSolver::writeSolution(const std::string& postfix) {
-  mkdirs(m_outputDir);
-  const std::string file = m_outputDir + m_solutionFile + "_" + std::to_string(m_timeStep) + postfix + ".csv";
-  std::ofstream out(file);
-  if (!out) {
-    std::cerr << "WARNING: cannot write " << file << std::endl;
+// LBMSolver::output (src/lbm/solver.cpp:323-382): every solution_interval
+// steps and when forced, the fields u, v, (w), rho of every cell that passes
+// the output filter go to <output_dir><solution_filename>_<step><postfix>.vtp;
+// after a divergence, once, the fields as they are with postfix "bdiv".
+void LBMSolver::output(bool forced, const std::string& postfix) {
+  if (m_diverged && !m_wroteDiverged) {
+    m_wroteDiverged = true;
+    output(true, "bdiv");
     return;
   }
-  out.precision(17);
-  out << (m_dim == 2 ? "cellId,x,y,u,v,rho\n" : "cellId,x,y,z,u,v,w,rho\n");
-  for (GInt q = 0; q < m_grid->size(); ++q) {
-    out << q;
-    for (GInt d = 0; d < m_dim; ++d) out << "," << m_grid->center(q, d);
-    for (GInt d = 0; d < m_dim; ++d) out << "," << m_u[q * m_dim + d];
-    out << "," << m_rho[q] << "\n";
+  if (!((m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || forced)) return;
+  fetchMacros();
+  const GInt N = m_grid->size();
+  const GInt n = m_outputAll ? N : 0;
+  if (n == 0) {
+    std::cerr << "ERROR: Invalid call to encodeLE() with length = 0" << std::endl;
+    std::exit(-1);
   }
+  static const char* velName[3] = {"U", "V", "W"};  // VAR::VELSTR, src/lbm/variables.h:70
+  std::vector<vtk::Field> fields;
+  for (GInt d = 0; d < m_dim; ++d) {
+    vtk::Field f{velName[d], std::vector<double>((size_t)N)};
+    for (GInt q = 0; q < N; ++q) f.values[q] = vtk::textRoundTrip(m_u[q * m_dim + d]);
+    fields.push_back(std::move(f));
+  }
+  vtk::Field r{"rho", std::vector<double>((size_t)N)};
+  for (GInt q = 0; q < N; ++q) r.values[q] = vtk::textRoundTrip(m_rho[q]);
+  fields.push_back(std::move(r));
+  std::vector<double> xyz((size_t)N * m_dim);
+  for (GInt q = 0; q < N; ++q)
+    for (GInt d = 0; d < m_dim; ++d) xyz[q * m_dim + d] = m_grid->center(q, d);
+  mkdirs(m_outputDir);
+  const std::string base = m_outputDir + m_solutionFile + "_" + std::to_string(m_timeStep) + postfix;
+  if (!vtk::writePoints(base, (int)m_dim, N, xyz.data(), fields))
+    std::cerr << "WARNING: cannot write " << base << ".vtp" << std::endl;
 }
 
 // `lbm --bench`: the reference's LBMSolver::initBenchmark is TERMM("Not

@@ -63,7 +63,7 @@ class LBMSolver : public Runnable {
   auto convergenceCondition(const lbm_sums& s) -> bool;
   void fetchMacros();
   void compareToAnalyticalResult();
-  void writeSolution(const std::string& postfix);
+  void output(bool forced, const std::string& postfix);
   auto runBenchmark() -> GInt;
   void check(int rc, const char* what);
 
@@ -85,7 +85,8 @@ class LBMSolver : public Runnable {
   std::array<GDouble, 3> m_force{{0, 0, 0}};
   int m_storage = LBM_STORE_F32;
   GInt m_timeStep = 0;
-  bool m_diverged = false;
+  bool m_diverged = false, m_wroteDiverged = false;
+  bool m_outputAll = true;  // the output filter passes every cell (else none)
   std::vector<GDouble> m_vars;  // host initial values [cell][dim+1]
   std::vector<GDouble> m_rho, m_u;
 };

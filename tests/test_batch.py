@@ -66,6 +66,58 @@ def test_batch_equals_single_domain_over_graph_replays(storage, tile):
     b.close()
 
 
+def interior_domains(dim):
+    """Domains with many interior tiles (no face, wrap or solid within one
+    cell), which run in the kernel without boundary paths: odd numbers of
+    interior tile rows, a solid block that splits columns of tiles, and
+    different widths."""
+    if dim == 3:
+        faces = {"-y": ("wall", (0.0, 0.0, 0.0), 1.0), "+y": ("wall", (0.03, 0.0, 0.0), 1.0)}
+        return [{"shape": (150, 30, 6), "omega": 1.4, "solid": None, "faces": faces},
+                {"shape": (70, 22, 5), "omega": 1.1, "solid": None, "faces": {}}]
+    doms = []
+    for k, (nx, ny) in enumerate([(200, 66), (131, 75), (97, 42)]):
+        solid = np.zeros((ny, nx), dtype=np.uint8)
+        if k == 1:
+            solid[30:36, 60:70] = 1
+        doms.append({"shape": (nx, ny), "omega": 1.0 + 0.3 * k, "solid": solid.reshape(-1),
+                     "faces": W.channel_faces(0.03)})
+    return doms
+
+
+@pytest.mark.parametrize("dim,storage,tile", [(2, "f32", 32), (2, "f32", 64), (2, "f64", 32), (2, "f16", 64),
+                                              (2, "bf16", 32), (3, "f32", 32), (3, "f32", 64)])
+def test_batch_interior_tiles_equal_single_domain(dim, storage, tile):
+    """Interior tiles (their own kernel) and boundary tiles together give
+    results bit-identical to the single-domain step, in every storage and in
+    3D."""
+    doms = interior_domains(dim)
+    rng = np.random.default_rng(11)
+    rhos, us = [], []
+    for d in doms:
+        n = int(np.prod(d["shape"]))
+        rhos.append(1 + 0.01 * rng.standard_normal(n))
+        us.append(0.02 * rng.standard_normal((n, dim)))
+    print(f"interior tiles {dim}D {storage} tile {tile}", flush=True)
+    b = capi.Batch(doms, storage=storage)
+    b.set_tiling(tile, 0)
+    b.init(np.concatenate(rhos), np.concatenate(us))
+    singles = []
+    for d, r, u in zip(doms, rhos, us):
+        lat = capi.Lattice(d["shape"], storage=storage, omega=d["omega"], faces=d["faces"] or None,
+                           solid=d["solid"])
+        lat.init(r, u)
+        singles.append(lat)
+    for replay in range(2):
+        b.run(8, 0.0, 8)
+        for k, lat in enumerate(singles):
+            lat.step(8)
+            assert np.array_equal(b.populations(k), lat.populations()), (replay, k)
+    for lat in singles:
+        lat.close()
+    b.close()
+
+
 def test_batch_against_oracle_f64():
     doms = small_domains()[:3]
     rhos, us = random_init(doms, seed=3)

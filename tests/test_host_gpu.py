@@ -52,6 +52,33 @@ def test_poiseuille_periodic_pressure_reference_thresholds_f64(tmp_path):
     assert "max. Error" in r.stderr
 
 
+def test_cli_writes_reference_vtp_output(tmp_path):
+    """LBMSolver::output (src/lbm/solver.cpp:194-195,323-382): a .vtp file at
+    every solution_interval step and at the last step, holding the points and
+    U, V, rho of every cell in cell-id order, the values after the 15-decimal
+    text round trip; the file bytes equal the writer's restatement."""
+    from oracle import vtk_oracle as V
+
+    cfg = json.load(open(os.path.join(REFCFG, "couette__couette.json")))
+    s = cfg["solver"]
+    s.pop("analyticalSolution", None)
+    s["maxSteps"], s["solution_interval"], s["convergence"] = 120, 50, 1e-300
+    r = run_cli(cfg, tmp_path, "-p", "f64")
+    assert r.returncode == 0, r.stderr[-2000:]
+    name = s.get("solution_filename", "solution")
+    files = sorted(os.listdir(tmp_path / "out"))
+    assert files == sorted(f"{name}_{t}.vtp" for t in (50, 100, 119)), files
+    (tmp_path / "again").mkdir()
+    h, rho, u, steps = host_run(json.loads(json.dumps(cfg)), tmp_path / "again", "f64")
+    data = (tmp_path / "out" / f"{name}_119.vtp").read_bytes()
+    pts, vals, headers = V.read_points(data)
+    assert np.array_equal(pts[:, :2], h["cen"].astype(np.float32))
+    want = {"U": u[:, 0], "V": u[:, 1], "rho": rho}
+    for k, v in want.items():
+        assert np.array_equal(vals[k], np.array([V.text_round_trip(x) for x in v])), k
+    assert data == V.write_points(2, h["cen"].tolist(), [(k, vals[k].tolist()) for k in ("U", "V", "rho")])
+
+
 WETNODE_REFCFG = ["couette__couette_bnd_eq.json", "couette__couette_bnd_eq2.json", "couette__couette_bnd_eq_aligned.json",
                   "couette__couette_bnd_NEEM.json", "couette__couette_bnd_NEBB.json", "poiseuille__poiseuille_bnd_eq.json",
                   "poiseuille__poiseuille_bnd_NEEM.json", "poiseuille__poiseuille_bnd_NEBB.json",
