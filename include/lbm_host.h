@@ -38,6 +38,12 @@ int lbm_host_grid_structured(const lbm_host_grid *g, int64_t *nxyz, int64_t *ind
 int32_t lbm_host_grid_surface_count(const lbm_host_grid *g);
 int lbm_host_grid_surface(const lbm_host_grid *g, int32_t i, char *name, int32_t name_len, int64_t *ncell,
                           int64_t *cells, int64_t *dirs);
+/* Cells of the postprocessing function "line" through a[dim], b[dim]
+ * (PostprocessCartesianFunctionLine::init,
+ * src/postprocess/postprocessing_cartesian.h:17-42): centre within half a cell
+ * length of the hyperplane through a and b, sorted by centre.  Writes at
+ * most cap ids to out; returns the number of cells, or -1. */
+int64_t lbm_host_grid_line_cells(const lbm_host_grid *g, const double *a, const double *b, int64_t *out, int64_t cap);
 void lbm_host_grid_destroy(lbm_host_grid *g);
 
 /* Run the solver on a configuration file like `lbm <file>`; storage may be

@@ -88,6 +88,16 @@ int lbm_host_grid_surface(const lbm_host_grid* g, int32_t i, char* name, int32_t
   return 0;
 }
 
+int64_t lbm_host_grid_line_cells(const lbm_host_grid* g, const double* a, const double* b, int64_t* out, int64_t cap) {
+  if (!g || !a || !b || cap < 0 || (cap > 0 && !out) || g->grid->dim() < 2) {
+    g_host_err = "lbm_host_grid_line_cells: invalid argument";
+    return -1;
+  }
+  const std::vector<GInt> cells = lineCells(*g->grid, a, b);
+  for (size_t i = 0; i < cells.size() && (int64_t)i < cap; ++i) out[i] = cells[i];
+  return (int64_t)cells.size();
+}
+
 void lbm_host_grid_destroy(lbm_host_grid* g) { delete g; }
 
 int lbm_host_run_file(const char* config_file, const char* storage, int64_t n, double* rho, double* u,

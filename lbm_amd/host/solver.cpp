@@ -123,7 +123,7 @@ void LBMSolver::loadConfiguration() {
       TERMM(-1, "Unknown output filter " + f);
     }
   }
-  c.getObject("postprocessing");
+  m_ppConf = c.getObject("postprocessing");
   c.opt_config_value<bool>("assumeAxisAligned", true);
   m_refLength = c.opt_config_value<GDouble>("refLength", 1.0);
   if (c.has_config_value("forcing"))
@@ -241,6 +241,17 @@ auto LBMSolver::run() -> GInt {
   check(lbm_create(&g, &p, &m_h), "lbm_create");
   check(lbm_set_bc(m_h, plan.faces.data(), (int32_t)plan.faces.size()), "lbm_set_bc");
   initialCondition();
+  // postprocessing (src/lbm/solver.cpp:187,199,309,318): the functions read
+  // the u held by the solver, which the reference refreshes at the start of
+  // every step (macros of the state before it) and when writing output
+  m_pp.init(m_ppConf, *m_grid);
+  {
+    const GInt N = m_grid->size(), nv = m_dim + 1;
+    m_ppU.assign(N, 0.0);
+    for (GInt q = 0; q < N; ++q) m_ppU[q] = m_vars[q * nv];
+  }
+  const bool ppStep = m_pp.active(PPHook::BeforeTimestep) || m_pp.active(PPHook::AfterTimestep);
+  m_pp.execute(PPHook::AtStart, *m_grid, m_ppU);
 
   const auto t0 = std::chrono::steady_clock::now();
   auto lastInfo = t0;
@@ -265,6 +276,11 @@ auto LBMSolver::run() -> GInt {
     const bool chk = (k + 1) % I == 0;
     const bool snap = (k + 2) % I == 0;
     const int mode = chk ? (snap ? LBM_STEP_CHECK_SNAPSHOT : LBM_STEP_CHECK) : (snap ? LBM_STEP_SNAPSHOT : LBM_STEP_PLAIN);
+    m_pp.execute(PPHook::BeforeTimestep, *m_grid, m_ppU);
+    if (ppStep) {
+      fetchMacros();
+      for (GInt q = 0; q < m_grid->size(); ++q) m_ppU[q] = m_u[q * m_dim];
+    }
     lbm_sums s{};
     const int rc = lbm_step_ex(m_h, mode, chk ? &s : nullptr);
     if (rc == LBM_DIVERGED) {
@@ -278,12 +294,14 @@ auto LBMSolver::run() -> GInt {
       }
     }
     // also at the last step or once converged (src/lbm/solver.cpp:194-195)
+    m_pp.execute(PPHook::AfterTimestep, *m_grid, m_ppU);
     output(m_timeStep == m_maxSteps - 1 || converged, "");
   }
   check(lbm_synchronize(m_h), "lbm_synchronize");
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
             << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
+  m_pp.execute(PPHook::AtEnd, *m_grid, m_ppU);
   fetchMacros();
   if (m_diverged) TERMM(-1, "Solution diverged");
   if (m_solverCfg->has_config_value("analyticalSolution")) compareToAnalyticalResult();
@@ -360,6 +378,7 @@ void LBMSolver::output(bool forced, const std::string& postfix) {
   if (!((m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || forced)) return;
   fetchMacros();
   const GInt N = m_grid->size();
+  for (GInt q = 0; q < N; ++q) m_ppU[q] = m_u[q * m_dim];
   const GInt n = m_outputAll ? N : 0;
   if (n == 0) {
     std::cerr << "ERROR: Invalid call to encodeLE() with length = 0" << std::endl;

@@ -8,6 +8,7 @@
 #include "../../include/lbm_hip.h"
 #include "bnd.h"
 #include "grid.h"
+#include "postprocess.h"
 
 namespace lbmh {
 
@@ -89,6 +90,9 @@ class LBMSolver : public Runnable {
   bool m_outputAll = true;  // the output filter passes every cell (else none)
   std::vector<GDouble> m_vars;  // host initial values [cell][dim+1]
   std::vector<GDouble> m_rho, m_u;
+  json m_ppConf;
+  Postprocess m_pp;
+  std::vector<GDouble> m_ppU;  // u component the postprocessing functions read
 };
 
 }  // namespace lbmh

@@ -79,6 +79,30 @@ def test_cli_writes_reference_vtp_output(tmp_path):
     assert data == V.write_points(2, h["cen"].tolist(), [(k, vals[k].tolist()) for k in ("U", "V", "rho")])
 
 
+def test_cli_line_postprocessing(tmp_path):
+    """The reference configs' "line" function at hook atEnd: line.csv in the
+    working directory with the centres and the final u of the cells along
+    the line (src/postprocess/postprocessing.h:88-116)."""
+    from oracle import post_oracle as P
+    from tests.test_postprocess import cell_length, host_line
+
+    cfg = json.load(open(os.path.join(REFCFG, "poiseuille__poiseuille_bnd.json")))
+    s = cfg["solver"]
+    s.pop("analyticalSolution", None)
+    s["maxSteps"], s["convergence"] = 300, 1e-300
+    r = run_cli(cfg, tmp_path, "-p", "f64")
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Executing postprocessing at hook:atEnd" in r.stderr
+    (tmp_path / "again").mkdir()
+    h, rho, u, steps = host_run(json.loads(json.dumps(cfg)), tmp_path / "again", "f64")
+    f = list(s["postprocessing"].values())[0]
+    cells = host_line(cfg, f["A"], f["B"])
+    assert np.array_equal(cells, P.line_cells(h["cen"], cell_length(h["cen"]), f["A"], f["B"]))
+    text = (tmp_path / "line.csv").read_text()
+    assert text == P.line_csv(h["cen"], cells, u[:, 0])
+    assert len(text.splitlines()) == len(cells) + 1 and len(cells) >= 32
+
+
 WETNODE_REFCFG = ["couette__couette_bnd_eq.json", "couette__couette_bnd_eq2.json", "couette__couette_bnd_eq_aligned.json",
                   "couette__couette_bnd_NEEM.json", "couette__couette_bnd_NEBB.json", "poiseuille__poiseuille_bnd_eq.json",
                   "poiseuille__poiseuille_bnd_NEEM.json", "poiseuille__poiseuille_bnd_NEBB.json",
@@ -133,7 +157,12 @@ def host_run(cfg, tmp_path, storage):
     rho = np.zeros(n)
     u = np.zeros(n * dim)
     steps = ct.c_int64()
-    rc = L.lbm_host_run_file(str(p).encode(), storage.encode(), n, rho.ctypes.data, u.ctypes.data, ct.byref(steps))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)  # postprocessing writes line.csv into the working directory
+    try:
+        rc = L.lbm_host_run_file(str(p).encode(), storage.encode(), n, rho.ctypes.data, u.ctypes.data, ct.byref(steps))
+    finally:
+        os.chdir(cwd)
     assert rc == 0
     return h, rho, u.reshape(n, dim), steps.value
 
