@@ -120,7 +120,9 @@ def test_structured_view_is_consistent():
     assert h["n"] < nx * ny * nz  # sphere cells removed
 
 
-def test_couette_cell_count_matches_survey_derivation():
+def test_couette_cell_count_matches_box_derivation():
+    """couette.json's box at its level: 20 x 32 cells (the faces sit on the
+    outer cell centres), 640 in all."""
     cfg = json.load(open(os.path.join(REFCFG, "couette__couette.json")))
     h = host_grid(cfg)
     assert h["n"] == 640 and list(h["nxyz"]) == [20, 32, 1]
