@@ -143,7 +143,9 @@ struct WgradArgs {
 };
 
 // One wave: (co tile, ci tile, tap, pixel chunk).  Operands are gathered
-// along the pixel axis (strided 2-byte loads).
+// along the pixel axis (strided 2-byte loads), the pixel coordinates carried
+// from pixel to pixel; the sum over the chunk runs in pixel order, so the
+// gradient is the same bit for bit however the loads are scheduled.
 __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ci = a.ca + a.cs;
@@ -170,8 +172,18 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (long long pk = pbeg; pk < pend; pk += 16) {
-    u16x8 av, bv;
+  // image, row and column of this lane's first pixel pk + 8 g, advanced by
+  // carries instead of divisions
+  int bx, by, bb;
+  {
+    const long long p0 = pbeg + 8 * g;
+    bx = (int)(p0 % a.wd);
+    by = (int)((p0 / a.wd) % a.h);
+    bb = (int)(p0 / ((long long)a.wd * a.h));
+  }
+  // operands of the 16 pixels from pk: this lane's 8 pixels pk + 8 g + i
+  auto gather = [&](long long pk, u16x8 &av, u16x8 &bv) {
+    int x = bx, y = by, b = bb;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const long long p = pk + 8 * g + i;
@@ -179,8 +191,6 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
       if (p < pend) {
         va = a.gy[p * a.co + co0 + m];
         if (cok) {
-          const int x = (int)(p % a.wd), y = (int)((p / a.wd) % a.h);
-          const int b = (int)(p / ((long long)a.wd * a.h));
           const int ys = y + dy, xs = x + dx;
           if (ys >= 0 && ys < a.h && xs >= 0 && xs < a.wd) {
             if (upart) vb = a.a[(((long long)b * h2 + (ys >> 1)) * w2 + (xs >> 1)) * a.ca + cin];
@@ -190,9 +200,36 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
       }
       av[i] = va;
       bv[i] = vb;
+      if (++x == a.wd) {
+        x = 0;
+        if (++y == a.h) {
+          y = 0;
+          ++b;
+        }
+      }
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+    bx += 16;  // the next step starts 16 pixels on
+    while (bx >= a.wd) {
+      bx -= a.wd;
+      if (++by == a.h) {
+        by = 0;
+        ++bb;
+      }
+    }
+  };
+  // two steps' loads in flight before their two products, which are
+  // accumulated in pixel order as one step at a time would (four steps in
+  // flight measured no faster)
+  for (long long pk = pbeg; pk < pend; pk += 32) {
+    u16x8 av0, bv0, av1, bv1;
+    gather(pk, av0, bv0);
+    const bool two = pk + 16 < pend;  // wave-uniform
+    if (two) gather(pk + 16, av1, bv1);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av0), __builtin_bit_cast(bf16x8, bv0),
                                                   acc, 0, 0, 0);
+    if (two)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av1), __builtin_bit_cast(bf16x8, bv1),
+                                                    acc, 0, 0, 0);
   }
   // C[mm][n]: mm = output channel row, n = input channel column (lane % 32)
   if (!cok) return;
@@ -224,7 +261,16 @@ __global__ void __launch_bounds__(256) k_bias_part(const u16 *gy, long long npix
   const long long beg = (long long)blockIdx.x * WCHUNK;
   const long long end = beg + WCHUNK < npix ? beg + WCHUNK : npix;
   float s = 0.f;
-  for (long long p = beg + r; p < end; p += rows) s += bf2f(gy[p * co + c]);
+  long long p = beg + r;
+  // eight loads in flight, added in the same order as one at a time
+  for (; p + 7LL * rows < end; p += 8LL * rows) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = bf2f(gy[(p + (long long)k * rows) * co + c]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; p < end; p += rows) s += bf2f(gy[p * co + c]);
   red[threadIdx.x] = s;
   __syncthreads();
   for (int k = rows / 2; k > 0; k >>= 1) {
