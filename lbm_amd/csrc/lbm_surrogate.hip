@@ -35,7 +35,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int WCHUNK = 4096;  // pixels per wgrad partial sum
+constexpr int WCHUNK = 4096;   // pixels per bias-gradient partial sum
+constexpr int WGCHUNK = 1024;  // pixels per weight-gradient partial sum (one wave each)
 
 __device__ __forceinline__ float bf2f(u16 b) { return __uint_as_float(((unsigned)b) << 16); }
 __device__ __forceinline__ u16 f2bf(float v) {
@@ -162,8 +163,8 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
   const int co0 = cot * 32, ci0 = cit * 32;
   const int m = lane & 31, g = lane >> 5;
   const long long npix = (long long)a.nb * a.h * a.wd;
-  const long long pbeg = (long long)chunk * WCHUNK;
-  const long long pend = pbeg + WCHUNK < npix ? pbeg + WCHUNK : npix;
+  const long long pbeg = (long long)chunk * WGCHUNK;
+  const long long pend = pbeg + WGCHUNK < npix ? pbeg + WGCHUNK : npix;
   const int dy = tap / 3 - 1, dx = tap % 3 - 1;
   const int cin = ci0 + m;  // this lane's B column (input channel)
   const bool cok = cin < ci;
@@ -246,7 +247,16 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long l
   const long long i = blockIdx.x * 256LL + threadIdx.x;
   if (i >= n) return;
   float s = 0.f;
-  for (int c = 0; c < nchunks; ++c) s += part[(long long)c * n + i];
+  int c = 0;
+  // eight loads in flight, added in chunk order
+  for (; c + 8 <= nchunks; c += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[(long long)(c + k) * n + i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; c < nchunks; ++c) s += part[(long long)c * n + i];
   dst[i] = s;
 }
 
@@ -547,7 +557,7 @@ static int alloc_all(lbm_sur *s) {
     SCHK(s, hipMalloc(&l.wb, n * sizeof(u16)));
     SCHK(s, hipMalloc(&l.wd, n * sizeof(u16)));
     const long long np = npix_at(s, s->t[l.s].level, s->B);
-    const long long nch = (np + WCHUNK - 1) / WCHUNK;
+    const long long nch = (np + WGCHUNK - 1) / WGCHUNK;  // WCHUNK >= WGCHUNK: covers the bias partials
     s->part_cap = std::max(s->part_cap, nch * n);
     if (l.ca > 0) s->gup_cap = std::max(s->gup_cap, np * l.ca);
   }
@@ -668,16 +678,17 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
     wa.nb = nb;
     wa.h = h;
     wa.wd = w;
-    wa.nchunks = (int)((np + WCHUNK - 1) / WCHUNK);
+    wa.nchunks = (int)((np + WGCHUNK - 1) / WGCHUNK);
     wa.part = s->part;
     const long long nw = (long long)l.co * 9 * ci;
     const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 9 * wa.nchunks;
     hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->st, wa);
     hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(nw)), dim3(256), 0, s->st, (const float *)s->part, nw, wa.nchunks,
                        s->gp + l.woff);
-    hipLaunchKernelGGL(k_bias_part, dim3(wa.nchunks), dim3(256), 0, s->st, gy, np, l.co, s->part);
+    const int bchunks = (int)((np + WCHUNK - 1) / WCHUNK);
+    hipLaunchKernelGGL(k_bias_part, dim3(bchunks), dim3(256), 0, s->st, gy, np, l.co, s->part);
     hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(l.co)), dim3(256), 0, s->st, (const float *)s->part, (long long)l.co,
-                       wa.nchunks, s->gp + l.boff);
+                       bchunks, s->gp + l.boff);
     // input gradient (not for the network input)
     if (li > 0) {
       ConvArgs a;

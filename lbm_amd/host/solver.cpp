@@ -103,6 +103,7 @@ void LBMSolver::loadConfiguration() {
   m_outputDir = c.opt_config_value<std::string>("output_dir", "out/");
   if (m_outputDir.empty() || m_outputDir.back() != '/') m_outputDir += '/';
   m_solutionInterval = c.opt_config_value<GInt>("solution_interval", 100);
+  if (m_solutionInterval < 1) TERMM(-1, "solution_interval must be >= 1");
   m_solutionFile = c.opt_config_value<std::string>("solution_filename", "solution");
   // output filter (CellFilterManager, src/cell_filter.h:64-103), default
   // {"cellFilter": "leafCells"} (src/lbm/solver.cpp:86-87); the grids of
@@ -226,6 +227,7 @@ auto LBMSolver::run() -> GInt {
   if (!m_grid) TERMM(-1, "No grid transferred to the LBM solver");
   std::cerr << m_grid->dim() << "D LBM Solver started ||>" << std::endl;
   loadConfiguration();
+  m_pp.init(m_ppConf, *m_grid);  // Postprocess::init (src/lbm/solver.cpp:180)
   const DevicePlan plan = m_bnd.lower(*m_grid);
   lbm_grid_desc g;
   g.dim = (int32_t)m_dim;
@@ -244,7 +246,6 @@ auto LBMSolver::run() -> GInt {
   // postprocessing (src/lbm/solver.cpp:187,199,309,318): the functions read
   // the u held by the solver, which the reference refreshes at the start of
   // every step (macros of the state before it) and when writing output
-  m_pp.init(m_ppConf, *m_grid);
   {
     const GInt N = m_grid->size(), nv = m_dim + 1;
     m_ppU.assign(N, 0.0);
