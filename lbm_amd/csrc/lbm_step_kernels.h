@@ -17,6 +17,19 @@ template <>
 struct StepWaves<19, float, 0> {
   static constexpr int value = 4;
 };
+// 16-bit storage (FP32 arithmetic), general kernel: 104 VGPRs unconstrained
+// (4 waves); held to 6 waves it spills 92-96 bytes per lane in its boundary
+// paths and still runs the 512x128x128 sphere channel 21-26 % faster (FP16
+// 273 -> 216 us, BF16 297 -> 240 us per step, identical results); 5 waves
+// gave 234 us (FP16); at 8 the register allocator gives up.
+template <>
+struct StepWaves<19, __half, 0> {
+  static constexpr int value = 6;
+};
+template <>
+struct StepWaves<19, __hip_bfloat16, 0> {
+  static constexpr int value = 6;
+};
 
 
 template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO>
