@@ -263,6 +263,38 @@ def run_warmstart(args):
     print(json.dumps(line), flush=True)
 
 
+def run_train(args):
+    """Training steps of the warm-start network (16 samples of 256x128,
+    forward + backward + Adam) on seeded random fields: W untimed steps, then
+    K timed steps, wall time around the synchronous training call."""
+    import numpy as np
+
+    from lbm_amd import surrogate as S
+    from lbm_amd import warmstart as WS
+
+    b = 16
+    sur = S.Surrogate(batch=b, height=WS.NY, width=WS.NX, seed=7)
+    rng = np.random.default_rng(1)
+    n = 4 * b
+    x = rng.standard_normal((n, WS.NY, WS.NX, S.IN_CHANNELS)).astype(np.float32)
+    t = rng.standard_normal((n, WS.NY, WS.NX, 3)).astype(np.float32)
+    m = np.ones((n, WS.NY, WS.NX), np.uint8)
+    sur.set_dataset(x, t, m)
+    idx = lambda k: np.resize(np.arange(n), k * b)
+    if args.warmup:
+        sur.train(idx(args.warmup), args.lr)
+    t0 = time.perf_counter()
+    losses = sur.train(idx(args.steps), args.lr)
+    sec = time.perf_counter() - t0
+    sur.close()
+    print(json.dumps({"metric": "warm-start network training step", "value": round(sec / args.steps * 1e3, 3),
+                      "unit": "ms/step", "higher_is_better": False, "steps": args.steps, "warmup": args.warmup,
+                      "samples_per_s": round(b * args.steps / sec, 1), "dtype": "bf16 MFMA, fp32 master weights",
+                      "data": "synthetic (seeded random fields)",
+                      "config": {"workload": f"U-Net, batch {b} x {WS.NY}x{WS.NX}, {S.IN_CHANNELS} input channels"},
+                      "loss_last": float(losses[-1])}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,8 +303,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tgv", choices=["tgv", "batch", "sphere", "warmstart"],
-                    help="tgv: the headline 256^3 box (default); batch / sphere / warmstart: time-to-residual workloads")
+    ap.add_argument("--workload", default="tgv", choices=["tgv", "batch", "sphere", "warmstart", "train"],
+                    help="tgv: the headline 256^3 box (default); batch / sphere / warmstart: time-to-residual "
+                         "workloads; train: training-step throughput of the warm-start network")
     ap.add_argument("--ntrain", type=int, default=1024)
     ap.add_argument("--ntest", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=30)
@@ -303,6 +336,8 @@ def main():
         return run_batch(args)
     if args.workload == "sphere":
         return run_sphere(args)
+    if args.workload == "train":
+        return run_train(args)
     if args.workload == "warmstart":
         return run_warmstart(args)
 
