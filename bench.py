@@ -4,8 +4,10 @@ Workload (N=1): 256^3 periodic box, D3Q19 BGK, tau = 0.6, FP32 populations
 (stored shifted, f - w), Taylor-Green vortex u0 = 0.05 plus seeded Gaussian
 noise sigma = 1e-3, rho = 1.  A step is one fused collide+stream launch over
 all 16.8M cells.  W warm-up steps, then K timed steps bracketed by
-synchronisation; the per-launch time of the step kernel comes from HIP
-events recorded on the stream the kernel is launched on.
+synchronisation, run without a break as five repetitions with HIP events
+recorded between them on the stream the kernel is launched on; the reported
+time per step is the median over the repetitions (min, max and the mean
+over all K steps beside it).
 
 Algorithmic bytes per lattice update: 2 * 19 * 4 = 152 B (one read and one
 write of every population); achieved = 152 B * cells / launch time.
@@ -44,6 +46,16 @@ def cpu_baseline(seconds_target=12.0):
     mlups = n ** 3 * steps / dt / 1e6
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": O.num_threads(), "kind": "port",
             "sample": f"FP64 OpenMP oracle, D3Q19 periodic TGV {n}^3, {steps} steps ({dt:.1f} s)"}
+
+
+def repetitions(k, r=5):
+    """Split K timed steps into r repetitions (all K steps are timed as one
+    uninterrupted run; the median per-step time over the repetitions is
+    reported).  Equal when r divides K, otherwise the last K mod r
+    repetitions take one step more."""
+    r = max(1, min(r, k))
+    base, extra = divmod(k, r)
+    return [base + (1 if i >= r - extra else 0) for i in range(r)]
 
 
 def load_traffic(name):
@@ -344,24 +356,24 @@ def main():
     import numpy as np
 
     from lbm_amd import capi
+    from lbm_amd import workloads as W
 
     n = args.n
     shape = (n, n, n)
-    # Taylor-Green + noise initial field, generated identically to the oracle's
-    from oracle.oracle import taylor_green_noise
-
-    rho0, u0 = taylor_green_noise(shape, 0.05, 1e-3, seed=1234)
+    rho0, u0 = W.taylor_green_noise(shape, 0.05, 1e-3, seed=1234)
     lat = capi.Lattice(shape, storage=args.storage, omega=1 / 0.6)
     lat.init(rho0, u0)
     del rho0, u0
     lat.step(args.warmup)
     lat.sync()
+    groups = repetitions(args.steps)
     t0 = time.perf_counter()
-    ms = lat.time_steps(args.steps)  # HIP events on the kernel's stream
+    group_ms = lat.time_step_groups(groups)  # HIP events on the kernel's stream between repetitions
     lat.sync()
     wall = time.perf_counter() - t0
+    per_step = group_ms / np.asarray(groups)
     cells = n ** 3
-    ms_per_step = ms / args.steps
+    ms_per_step = float(np.median(per_step))
     mlups = cells / (ms_per_step * 1e-3) / 1e6
     bpp = capi.BYTES[args.storage]
     alg_bytes = 2 * 19 * bpp * cells
@@ -387,6 +399,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes},
+        "timing": {"repetitions": len(groups), "steps_per_repetition": groups,
+                   "ms_per_step_median": round(ms_per_step, 5), "ms_per_step_min": round(float(per_step.min()), 5),
+                   "ms_per_step_max": round(float(per_step.max()), 5),
+                   "ms_per_step_mean": round(float(group_ms.sum()) / args.steps, 5)},
         "wall_s_timed": round(wall, 4),
         "mass_after": s.mass,
     }

@@ -187,6 +187,12 @@ int lbm_set_storage(lbm_handle *h, int32_t storage);
 /* Time nsteps steps on the handle's stream with HIP events (ms total). */
 int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms);
 
+/* Time ngroups consecutive groups of steps (steps[g] steps each) as one
+ * uninterrupted run on the handle's stream: an event is recorded between
+ * groups, the host waits once at the end; ms[g] = time of group g.  Used to
+ * report the median of equal repetitions inside one timed run. */
+int lbm_time_step_groups(lbm_handle *h, const int32_t *steps, int32_t ngroups, double *ms);
+
 /* Launch shape of the step kernel: block_x threads along x (multiple of 64),
  * block_y rows (0 keeps the current value); nontemporal loads/stores on (1)
  * or off (0), -1 keeps the current setting. */

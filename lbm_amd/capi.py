@@ -49,7 +49,7 @@ class Sums(ct.Structure):
 
 # every symbol declared in include/lbm_hip.h
 EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_get_macros",
-           "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_set_launch",
+           "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_time_step_groups", "lbm_set_launch",
            "lbm_set_index_width", "lbm_set_storage", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
 _lib = None
@@ -76,6 +76,7 @@ def load():
     lib.lbm_get_populations.argtypes = [P, dp]
     lib.lbm_set_populations.argtypes = [P, dp]
     lib.lbm_time_steps.argtypes = [P, ct.c_int32, dp]
+    lib.lbm_time_step_groups.argtypes = [P, ct.POINTER(ct.c_int32), ct.c_int32, dp]
     lib.lbm_set_launch.argtypes = [P, ct.c_int32, ct.c_int32, ct.c_int32]
     lib.lbm_set_index_width.argtypes = [P, ct.c_int32]
     lib.lbm_set_storage.argtypes = [P, ct.c_int32]
@@ -203,6 +204,15 @@ class Lattice:
         ms = ct.c_double()
         self._check(self.lib.lbm_time_steps(self.h, int(n), ct.byref(ms)))
         return ms.value
+
+    def time_step_groups(self, groups):
+        """Time consecutive groups of steps in one uninterrupted run; returns
+        the milliseconds of each group."""
+        g = np.ascontiguousarray(np.asarray(groups, dtype=np.int32))
+        ms = np.zeros(g.size)
+        self._check(self.lib.lbm_time_step_groups(self.h, g.ctypes.data_as(ct.POINTER(ct.c_int32)), int(g.size),
+                                                  _dptr(ms)))
+        return ms
 
     def set_launch(self, bx=0, by=0, nt=-1):
         self._check(self.lib.lbm_set_launch(self.h, int(bx), int(by), int(nt)))

@@ -100,16 +100,33 @@ static int bfail(lbm_batch *b, int code, const std::string &msg) {
       return bfail((b), LBM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
+// The instantiated graph captures the tile list, the tile ids and the
+// partial sums among its kernel arguments, and HIP keeps a reference to
+// each argument's allocation in the captured kernel nodes; the graph must
+// therefore be destroyed before any of them is freed, never after.
+static void batch_drop_graph(lbm_batch *b) {
+  if (b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+}
+
 // Tiles of every still-active domain.  A tile needs the flag field when a
-// solid cell lies within one cell of it (its pull sources or itself).
+// solid cell lies within one cell of it (its pull sources or itself).  The
+// new buffers are allocated and filled before the old ones are freed, so a
+// failed rebuild leaves the previous tiling in place (without a graph; the
+// next run captures a new one).
 static int batch_build_tiles(lbm_batch *b, int tx) {
   const int ty = 256 / tx;
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);
   std::vector<int4> tiles;
+  std::vector<DomainDev> doms = b->doms;
   // interior tiles: full, no face, wrap or solid within one cell of them;
   // they run in a kernel compiled without the boundary paths
   std::vector<int> ids_int, ids_bnd;
   for (int d = 0; d < b->ndom; ++d) {
-    DomainDev &D = b->doms[d];
+    DomainDev &D = doms[d];
     D.tile_begin = (int)tiles.size();
     if (!b->active_h.empty() && !b->active_h[d]) {
       D.tile_end = D.tile_begin;
@@ -142,28 +159,36 @@ static int batch_build_tiles(lbm_batch *b, int tx) {
     ids_bnd.push_back(0);
     tiles.push_back(make_int4(0, 0, 0, 0));
   }
-  if (b->tiles_d) BHIPCHK(b, hipFree(b->tiles_d));
-  b->tiles_d = nullptr;
-  b->ntiles = (long long)tiles.size();
-  BHIPCHK(b, hipMalloc(&b->tiles_d, tiles.size() * sizeof(int4)));
-  BHIPCHK(b, hipMemcpy(b->tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
   std::vector<int> ids(ids_int);
   ids.insert(ids.end(), ids_bnd.begin(), ids_bnd.end());
-  if (b->ids_d) BHIPCHK(b, hipFree(b->ids_d));
-  b->ids_d = nullptr;
-  BHIPCHK(b, hipMalloc(&b->ids_d, ids.size() * sizeof(int)));
-  BHIPCHK(b, hipMemcpy(b->ids_d, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice));
+  int4 *tiles_d = nullptr;
+  int *ids_d = nullptr;
+  double *partials = nullptr;
+  auto release = [&](int code, const std::string &msg) {
+    if (tiles_d) (void)hipFree(tiles_d);
+    if (ids_d) (void)hipFree(ids_d);
+    if (partials) (void)hipFree(partials);
+    return bfail(b, code, msg);
+  };
+  if (hipMalloc(&tiles_d, tiles.size() * sizeof(int4)) != hipSuccess ||
+      hipMalloc(&ids_d, ids.size() * sizeof(int)) != hipSuccess ||
+      hipMalloc(&partials, tiles.size() * NSUM * sizeof(double)) != hipSuccess)
+    return release(LBM_ENOMEM, "batch tiling: device allocation failed");
+  if (hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ids_d, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->doms_d, doms.data(), doms.size() * sizeof(DomainDev), hipMemcpyHostToDevice) != hipSuccess)
+    return release(LBM_EHIP, "batch tiling: upload failed");
+  if (b->tiles_d) (void)hipFree(b->tiles_d);
+  if (b->ids_d) (void)hipFree(b->ids_d);
+  if (b->partials) (void)hipFree(b->partials);
+  b->tiles_d = tiles_d;
+  b->ids_d = ids_d;
+  b->partials = partials;
+  b->doms = std::move(doms);
+  b->ntiles = (long long)tiles.size();
   b->n_int = (int)ids_int.size();
   b->n_bnd = (int)ids_bnd.size();
-  BHIPCHK(b, hipMemcpy(b->doms_d, b->doms.data(), b->doms.size() * sizeof(DomainDev), hipMemcpyHostToDevice));
-  if (b->partials) BHIPCHK(b, hipFree(b->partials));
-  b->partials = nullptr;
-  BHIPCHK(b, hipMalloc(&b->partials, (size_t)b->ntiles * NSUM * sizeof(double)));
   b->tx = tx;
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
   return LBM_OK;
 }
 
@@ -348,9 +373,9 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) ncu = prop.multiProcessorCount;
   }
-  hipEvent_t e0, e1;
-  BHIPCHK(b, hipEventCreate(&e0));
-  BHIPCHK(b, hipEventCreate(&e1));
+  EventPair ev;
+  BHIPCHK(b, ev.create());
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   double best = 1e30;
   int best_tx = 64, best_blocks = 0;
   const int txs[4] = {32, 64, 128, 256};
@@ -374,8 +399,6 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   b->blocks = best_blocks;
   int rc = batch_build_tiles(b, best_tx);
   if (rc) return rc;
@@ -385,9 +408,10 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
 
 int lbm_batch_init_fields(lbm_batch *b, const double *rho, const double *u) {
   if (!b || !rho || !u) return bfail(b, LBM_EINVAL, "lbm_batch_init_fields: null argument");
-  double *rd = nullptr, *ud = nullptr;
-  BHIPCHK(b, hipMalloc(&rd, b->compact_cells * sizeof(double)));
-  BHIPCHK(b, hipMalloc(&ud, b->compact_cells * b->dim * sizeof(double)));
+  DevBuf rbuf, ubuf;
+  BHIPCHK(b, rbuf.alloc(b->compact_cells * sizeof(double)));
+  BHIPCHK(b, ubuf.alloc(b->compact_cells * b->dim * sizeof(double)));
+  double *rd = static_cast<double *>(rbuf.p), *ud = static_cast<double *>(ubuf.p);
   BHIPCHK(b, hipMemcpy(rd, rho, b->compact_cells * sizeof(double), hipMemcpyHostToDevice));
   BHIPCHK(b, hipMemcpy(ud, u, b->compact_cells * b->dim * sizeof(double), hipMemcpyHostToDevice));
   long long coff = 0;
@@ -417,16 +441,11 @@ int lbm_batch_init_fields(lbm_batch *b, const double *rho, const double *u) {
   hipLaunchKernelGGL(k_fill_ll, dim3((b->ndom + 255) / 256), dim3(256), 0, b->stream, b->conv_d, b->ndom, -1LL);
   hipLaunchKernelGGL(k_fill_ll, dim3(1), dim3(64), 0, b->stream, b->cycle_d, 1, 0LL);
   BHIPCHK(b, hipStreamSynchronize(b->stream));
-  BHIPCHK(b, hipFree(rd));
-  BHIPCHK(b, hipFree(ud));
   return LBM_OK;
 }
 
 static int batch_capture(lbm_batch *b, int interval, double tol) {
-  if (b->graph) {
-    (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-  }
+  batch_drop_graph(b);
   hipGraph_t g;
   BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
   for (int s = 0; s < interval; ++s) {
@@ -452,9 +471,9 @@ int lbm_batch_run(lbm_batch *b, int32_t interval, double tol, int64_t max_steps,
     if (rc) return rc;
   }
   std::vector<int> active(b->ndom);
-  hipEvent_t e0, e1;
-  BHIPCHK(b, hipEventCreate(&e0));
-  BHIPCHK(b, hipEventCreate(&e1));
+  EventPair ev;
+  BHIPCHK(b, ev.create());
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   BHIPCHK(b, hipEventRecord(e0, b->stream));
   long long done = 0;
   int tiled_active = 0;
@@ -482,8 +501,6 @@ int lbm_batch_run(lbm_batch *b, int32_t interval, double tol, int64_t max_steps,
   float t = 0;
   BHIPCHK(b, hipEventElapsedTime(&t, e0, e1));
   if (ms) *ms = t;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (results) {
     std::vector<double> res(b->ndom), mass(b->ndom);
     std::vector<long long> conv(b->ndom);
@@ -505,9 +522,10 @@ int lbm_batch_get_macros(lbm_batch *b, int32_t dom, double *rho, double *u) {
   if (!b || dom < 0 || dom >= b->ndom || !rho || !u) return bfail(b, LBM_EINVAL, "lbm_batch_get_macros: bad argument");
   const DomainDev &D = b->doms[dom];
   const long long n = (long long)D.nx * D.ny * D.nz;
-  double *rd = nullptr, *ud = nullptr;
-  BHIPCHK(b, hipMalloc(&rd, n * sizeof(double)));
-  BHIPCHK(b, hipMalloc(&ud, n * b->dim * sizeof(double)));
+  DevBuf rbuf, ubuf;
+  BHIPCHK(b, rbuf.alloc(n * sizeof(double)));
+  BHIPCHK(b, ubuf.alloc(n * b->dim * sizeof(double)));
+  double *rd = static_cast<double *>(rbuf.p), *ud = static_cast<double *>(ubuf.p);
   KArgs a;
   memset(&a, 0, sizeof(a));
   a.src = static_cast<char *>(b->buf[b->cur]) + D.pop_off * b->esize;
@@ -520,8 +538,6 @@ int lbm_batch_get_macros(lbm_batch *b, int32_t dom, double *rho, double *u) {
   BHIPCHK(b, hipMemcpyAsync(rho, rd, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   BHIPCHK(b, hipMemcpyAsync(u, ud, n * b->dim * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   BHIPCHK(b, hipStreamSynchronize(b->stream));
-  BHIPCHK(b, hipFree(rd));
-  BHIPCHK(b, hipFree(ud));
   return LBM_OK;
 }
 
@@ -529,8 +545,9 @@ int lbm_batch_get_populations(lbm_batch *b, int32_t dom, double *f) {
   if (!b || dom < 0 || dom >= b->ndom || !f) return bfail(b, LBM_EINVAL, "lbm_batch_get_populations: bad argument");
   const DomainDev &D = b->doms[dom];
   const long long n = (long long)D.nx * D.ny * D.nz;
-  double *fd = nullptr;
-  BHIPCHK(b, hipMalloc(&fd, n * b->Q * sizeof(double)));
+  DevBuf fbuf;
+  BHIPCHK(b, fbuf.alloc(n * b->Q * sizeof(double)));
+  double *fd = static_cast<double *>(fbuf.p);
   KArgs a;
   memset(&a, 0, sizeof(a));
   a.src = static_cast<char *>(b->buf[b->cur]) + D.pop_off * b->esize;
@@ -541,7 +558,6 @@ int lbm_batch_get_populations(lbm_batch *b, int32_t dom, double *f) {
   BHIPCHK(b, hipGetLastError());
   BHIPCHK(b, hipMemcpyAsync(f, fd, n * b->Q * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   BHIPCHK(b, hipStreamSynchronize(b->stream));
-  BHIPCHK(b, hipFree(fd));
   return LBM_OK;
 }
 

@@ -231,6 +231,8 @@ struct lbm_handle {
   double *partials = nullptr;
   long long partials_cap = 0;
   double *sums_d = nullptr;
+  void *io = nullptr;  // FP64 (rho, u) staging, see ensure_io
+  size_t io_cap = 0;
   hipStream_t stream = nullptr;
   int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
   bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
@@ -252,6 +254,55 @@ static int fail(lbm_handle *h, int code, const std::string &msg) {
     if (e_ != hipSuccess)                                                                    \
       return fail((h), LBM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
   } while (0)
+
+// A pair of timing events, destroyed on every exit path.
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  EventPair() = default;
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+  hipError_t create() {
+    for (hipEvent_t &x : e) {
+      const hipError_t r = hipEventCreate(&x);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  }
+  ~EventPair() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+// A transient device buffer, freed on every exit path unless released.
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+  void *release() {
+    void *q = p;
+    p = nullptr;
+    return q;
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Staging buffer for the FP64 (rho, u) fields of lbm_init_fields and
+// lbm_get_macros, kept by the handle so that repeated read-backs (output
+// and postprocessing hooks) do not allocate.
+static int ensure_io(lbm_handle *h, size_t bytes) {
+  if (bytes <= h->io_cap) return LBM_OK;
+  if (h->io) HIPCHK(h, hipFree(h->io));
+  h->io = nullptr;
+  h->io_cap = 0;
+  HIPCHK(h, hipMalloc(&h->io, bytes));
+  h->io_cap = bytes;
+  return LBM_OK;
+}
 
 static int upload_faces(lbm_handle *h) {
   HIPCHK(h, hipMemcpyAsync(h->dface, h->face, sizeof(h->face), hipMemcpyHostToDevice, h->stream));
@@ -528,12 +579,12 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
 
 int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   if (!h || !rho || !u) return fail(h, LBM_EINVAL, "lbm_init_fields: null argument");
-  double *rd = nullptr, *ud = nullptr;
-  HIPCHK(h, hipMalloc(&rd, h->ncell * sizeof(double)));
-  HIPCHK(h, hipMalloc(&ud, h->ncell * h->dim * sizeof(double)));
+  int rc = ensure_io(h, (size_t)h->ncell * (1 + h->dim) * sizeof(double));
+  if (rc) return rc;
+  double *rd = static_cast<double *>(h->io), *ud = rd + h->ncell;
   HIPCHK(h, hipMemcpyAsync(rd, rho, h->ncell * sizeof(double), hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync(ud, u, h->ncell * h->dim * sizeof(double), hipMemcpyHostToDevice, h->stream));
-  int rc = ensure_snapshots(h);
+  rc = ensure_snapshots(h);
   if (rc) return rc;
   KArgs a = make_args(h);
   a.dst = h->buf[h->cur];
@@ -544,8 +595,6 @@ int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   dispatch_qs<InitL>(h->Q, h->storage, a, g, b, h->stream, (const double *)rd, (const double *)ud);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipFree(rd));
-  HIPCHK(h, hipFree(ud));
   h->steps = 0;
   return LBM_OK;
 }
@@ -647,9 +696,9 @@ int lbm_step_ex(lbm_handle *h, int32_t mode, lbm_sums *out) {
 
 int lbm_get_macros(lbm_handle *h, double *rho, double *u) {
   if (!h || !rho || !u) return fail(h, LBM_EINVAL, "lbm_get_macros: null argument");
-  double *rd = nullptr, *ud = nullptr;
-  HIPCHK(h, hipMalloc(&rd, h->ncell * sizeof(double)));
-  HIPCHK(h, hipMalloc(&ud, h->ncell * h->dim * sizeof(double)));
+  const int rc = ensure_io(h, (size_t)h->ncell * (1 + h->dim) * sizeof(double));
+  if (rc) return rc;
+  double *rd = static_cast<double *>(h->io), *ud = rd + h->ncell;
   KArgs a = make_args(h);
   dim3 g, b;
   launch_dims(h, g, b);
@@ -658,8 +707,6 @@ int lbm_get_macros(lbm_handle *h, double *rho, double *u) {
   HIPCHK(h, hipMemcpyAsync(rho, rd, h->ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(u, ud, h->ncell * h->dim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipFree(rd));
-  HIPCHK(h, hipFree(ud));
   return LBM_OK;
 }
 
@@ -686,99 +733,109 @@ int lbm_reduce(lbm_handle *h, lbm_sums *out) {
 
 int lbm_get_populations(lbm_handle *h, double *f) {
   if (!h || !f) return fail(h, LBM_EINVAL, "lbm_get_populations: null argument");
-  double *fd = nullptr;
   const size_t bytes = (size_t)h->ncell * h->Q * sizeof(double);
-  HIPCHK(h, hipMalloc(&fd, bytes));
+  DevBuf fd;
+  HIPCHK(h, fd.alloc(bytes));
   KArgs a = make_args(h);
   dim3 g, b;
   launch_dims(h, g, b);
-  dispatch_qs<GetL>(h->Q, h->storage, a, g, b, h->stream, fd);
+  dispatch_qs<GetL>(h->Q, h->storage, a, g, b, h->stream, static_cast<double *>(fd.p));
   HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipMemcpyAsync(f, fd, bytes, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(f, fd.p, bytes, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipFree(fd));
   return LBM_OK;
 }
 
 int lbm_set_populations(lbm_handle *h, const double *f) {
   if (!h || !f) return fail(h, LBM_EINVAL, "lbm_set_populations: null argument");
-  double *fd = nullptr;
   const size_t bytes = (size_t)h->ncell * h->Q * sizeof(double);
-  HIPCHK(h, hipMalloc(&fd, bytes));
-  HIPCHK(h, hipMemcpyAsync(fd, f, bytes, hipMemcpyHostToDevice, h->stream));
+  DevBuf fd;
+  HIPCHK(h, fd.alloc(bytes));
+  HIPCHK(h, hipMemcpyAsync(fd.p, f, bytes, hipMemcpyHostToDevice, h->stream));
   KArgs a = make_args(h);
   a.dst = h->buf[h->cur];
   dim3 g, b;
   launch_dims(h, g, b);
-  dispatch_qs<SetL>(h->Q, h->storage, a, g, b, h->stream, (const double *)fd);
+  dispatch_qs<SetL>(h->Q, h->storage, a, g, b, h->stream, static_cast<const double *>(fd.p));
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipFree(fd));
   return LBM_OK;
 }
 
+// The new population and snapshot buffers are allocated and filled before
+// the old ones are released: on any failure the handle keeps its storage
+// form and state unchanged.
 int lbm_set_storage(lbm_handle *h, int32_t storage) {
   if (!h) return fail(h, LBM_EINVAL, "lbm_set_storage: null handle");
   const int es = esize_of(storage);
   if (!es) return fail(h, LBM_EINVAL, "lbm_set_storage: unknown storage");
   if (storage == h->storage) return LBM_OK;
-  if ((unsigned long long)h->plane * h->Q * es > (unsigned long long)h->plane * h->Q * h->esize) {
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
-        (unsigned long long)free_b < (unsigned long long)h->plane * h->Q * (2ull * es) + h->ncell * h->Q * 8ull)
-      return fail(h, LBM_ENOMEM, "lbm_set_storage: not enough device memory for the conversion");
+  const size_t bytes = (size_t)h->plane * h->Q * es;
+  const size_t sbytes = (size_t)h->px * h->ny * h->nz * snap_bytes(storage);
+  DevBuf fd, nb[2], ns[2];
+  if (fd.alloc((size_t)h->ncell * h->Q * sizeof(double)) != hipSuccess || nb[0].alloc(bytes) != hipSuccess ||
+      nb[1].alloc(bytes) != hipSuccess || ns[0].alloc(sbytes) != hipSuccess || ns[1].alloc(sbytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, LBM_ENOMEM, "lbm_set_storage: not enough device memory for the conversion (storage unchanged)");
   }
+  for (int k = 0; k < 2; ++k) HIPCHK(h, hipMemsetAsync(nb[k].p, 0, bytes, h->stream));
   // current state -> FP64 f (exact for every storage form) -> new storage
-  double *fd = nullptr;
-  HIPCHK(h, hipMalloc(&fd, (size_t)h->ncell * h->Q * sizeof(double)));
   KArgs a = make_args(h);
   dim3 g, b;
   launch_dims(h, g, b);
-  dispatch_qs<GetL>(h->Q, h->storage, a, g, b, h->stream, fd);
+  dispatch_qs<GetL>(h->Q, h->storage, a, g, b, h->stream, static_cast<double *>(fd.p));
+  HIPCHK(h, hipGetLastError());
+  a.dst = nb[h->cur].p;
+  dispatch_qs<SetL>(h->Q, storage, a, g, b, h->stream, static_cast<const double *>(fd.p));
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (int k = 0; k < 2; ++k) {
-    HIPCHK(h, hipFree(h->buf[k]));
-    h->buf[k] = nullptr;
-    if (h->snap[k]) HIPCHK(h, hipFree(h->snap[k]));
-    h->snap[k] = nullptr;
-  }
-  const size_t bytes = (size_t)h->plane * h->Q * es;
-  for (int k = 0; k < 2; ++k) {
-    if (hipMalloc(&h->buf[k], bytes) != hipSuccess) {
-      (void)hipFree(fd);
-      return fail(h, LBM_ENOMEM, "lbm_set_storage: hipMalloc of populations failed");
-    }
-    HIPCHK(h, hipMemsetAsync(h->buf[k], 0, bytes, h->stream));
+    (void)hipFree(h->buf[k]);
+    h->buf[k] = nb[k].release();
+    if (h->snap[k]) (void)hipFree(h->snap[k]);
+    h->snap[k] = ns[k].release();
   }
   h->storage = storage;
   h->esize = es;
   h->w64 = (unsigned long long)h->plane * (unsigned long long)es > 0xffffffffull;
   h->snap_last = 0;
-  a = make_args(h);
-  a.dst = h->buf[h->cur];
-  dispatch_qs<SetL>(h->Q, h->storage, a, g, b, h->stream, (const double *)fd);
-  HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  HIPCHK(h, hipFree(fd));
-  return ensure_snapshots(h);
+  return LBM_OK;
 }
 
 int lbm_time_steps(lbm_handle *h, int32_t nsteps, double *ms) {
   if (!h || !ms || nsteps <= 0) return fail(h, LBM_EINVAL, "lbm_time_steps: bad argument");
-  hipEvent_t e0, e1;
-  HIPCHK(h, hipEventCreate(&e0));
-  HIPCHK(h, hipEventCreate(&e1));
-  HIPCHK(h, hipEventRecord(e0, h->stream));
-  int rc = lbm_step(h, nsteps);
+  EventPair ev;
+  HIPCHK(h, ev.create());
+  HIPCHK(h, hipEventRecord(ev.e[0], h->stream));
+  const int rc = lbm_step(h, nsteps);
   if (rc) return rc;
-  HIPCHK(h, hipEventRecord(e1, h->stream));
-  HIPCHK(h, hipEventSynchronize(e1));
+  HIPCHK(h, hipEventRecord(ev.e[1], h->stream));
+  HIPCHK(h, hipEventSynchronize(ev.e[1]));
   float t = 0;
-  HIPCHK(h, hipEventElapsedTime(&t, e0, e1));
+  HIPCHK(h, hipEventElapsedTime(&t, ev.e[0], ev.e[1]));
   *ms = t;
-  HIPCHK(h, hipEventDestroy(e0));
-  HIPCHK(h, hipEventDestroy(e1));
+  return LBM_OK;
+}
+
+int lbm_time_step_groups(lbm_handle *h, const int32_t *steps, int32_t ngroups, double *ms) {
+  if (!h || !steps || !ms || ngroups < 1 || ngroups > 4096) return fail(h, LBM_EINVAL, "lbm_time_step_groups: bad argument");
+  for (int g = 0; g < ngroups; ++g)
+    if (steps[g] < 1) return fail(h, LBM_EINVAL, "lbm_time_step_groups: every group needs at least one step");
+  std::vector<EventPair> ev((ngroups + 2) / 2);
+  auto event = [&](int k) { return ev[k / 2].e[k % 2]; };
+  for (EventPair &p : ev) HIPCHK(h, p.create());
+  HIPCHK(h, hipEventRecord(event(0), h->stream));
+  for (int g = 0; g < ngroups; ++g) {
+    const int rc = lbm_step(h, steps[g]);
+    if (rc) return rc;
+    HIPCHK(h, hipEventRecord(event(g + 1), h->stream));
+  }
+  HIPCHK(h, hipEventSynchronize(event(ngroups)));
+  for (int g = 0; g < ngroups; ++g) {
+    float t = 0;
+    HIPCHK(h, hipEventElapsedTime(&t, event(g), event(g + 1)));
+    ms[g] = t;
+  }
   return LBM_OK;
 }
 
@@ -832,6 +889,7 @@ int lbm_destroy(lbm_handle *h) {
     if (h->snap[b]) hipFree(h->snap[b]);
   if (h->partials) hipFree(h->partials);
   if (h->sums_d) hipFree(h->sums_d);
+  if (h->io) hipFree(h->io);
   if (h->dface) hipFree(h->dface);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -843,29 +901,24 @@ const char *lbm_last_error(const lbm_handle *h) { return h ? h->err.c_str() : g_
 int lbm_copy_bench(int64_t nbytes, int32_t reps, double *ms) {
   if (nbytes < 16 || reps < 1 || !ms) return fail(nullptr, LBM_EINVAL, "lbm_copy_bench: bad argument");
   const long long n = nbytes / 16;
-  float4 *a = nullptr, *b = nullptr;
-  HIPCHK(nullptr, hipMalloc(&a, n * 16));
-  HIPCHK(nullptr, hipMalloc(&b, n * 16));
-  HIPCHK(nullptr, hipMemset(a, 0, n * 16));
-  hipStream_t s;
-  HIPCHK(nullptr, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  DevBuf a, b;
+  HIPCHK(nullptr, a.alloc(n * 16));
+  HIPCHK(nullptr, b.alloc(n * 16));
+  HIPCHK(nullptr, hipMemset(a.p, 0, n * 16));
+  EventPair ev;
+  HIPCHK(nullptr, ev.create());
+  const float4 *in = static_cast<const float4 *>(a.p);
+  float4 *out = static_cast<float4 *>(b.p);
   const long long grid = (n + 255) / 256;
-  hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, a, b, n);
-  hipEvent_t e0, e1;
-  HIPCHK(nullptr, hipEventCreate(&e0));
-  HIPCHK(nullptr, hipEventCreate(&e1));
-  HIPCHK(nullptr, hipEventRecord(e0, s));
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, a, b, n);
-  HIPCHK(nullptr, hipEventRecord(e1, s));
-  HIPCHK(nullptr, hipEventSynchronize(e1));
+  hipStream_t s = nullptr;  // the null stream: no stream to destroy on an error path
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, in, out, n);
+  HIPCHK(nullptr, hipEventRecord(ev.e[0], s));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3((unsigned)grid), dim3(256), 0, s, in, out, n);
+  HIPCHK(nullptr, hipEventRecord(ev.e[1], s));
+  HIPCHK(nullptr, hipEventSynchronize(ev.e[1]));
   float t = 0;
-  HIPCHK(nullptr, hipEventElapsedTime(&t, e0, e1));
+  HIPCHK(nullptr, hipEventElapsedTime(&t, ev.e[0], ev.e[1]));
   *ms = t / reps;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipStreamDestroy(s);
-  hipFree(a);
-  hipFree(b);
   return LBM_OK;
 }
 

@@ -8,10 +8,70 @@ pressure outlet rho = 1 at the right end, bounce-back top/bottom walls and step.
 
 obstacle_channel: 256 x 128 D2Q9 channel with 1..4 seeded circles or
 rectangles (radius / half-size 4..20) at Re 20..200 (Re on the channel height
-with inlet u = 0.05), Zou-He inlet, anti-bounce-back pressure outlet,
-bounce-back walls.
+with inlet u = 0.05), NEEM (non-equilibrium extrapolation) velocity inlet
+and pressure outlet, bounce-back walls.
+
+taylor_green_noise: the periodic-box start field, a Taylor-Green vortex at
+cell centres plus seeded Gaussian noise (splitmix64 + Box-Muller).
 """
 import numpy as np
+
+_MASK64 = (1 << 64) - 1
+
+
+def _splitmix64(first, count, seed):
+    """Outputs first+1 .. first+count of splitmix64 (Steele, Lea & Flood,
+    "Fast splittable pseudorandom number generators", OOPSLA 2014) started
+    from seed: state_k = seed + k * golden gamma, then the variant-13 mix."""
+    k = np.arange(first + 1, first + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & _MASK64) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _box_muller(first_pair, npairs, seed):
+    """Standard normal pairs: outputs 2p+1 and 2p+2 of the generator give
+    u1 in (0, 1] and u2 in [0, 1) from their top 53 bits; the pair is
+    sqrt(-2 ln u1) (cos 2 pi u2, sin 2 pi u2)."""
+    r = _splitmix64(2 * first_pair, 2 * npairs, seed)
+    scale = 1.0 / 9007199254740992.0  # 2^-53
+    u1 = ((r[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * scale
+    u2 = (r[1::2] >> np.uint64(11)).astype(np.float64) * scale
+    rad = np.sqrt(-2.0 * np.log(u1))
+    out = np.empty(2 * npairs)
+    out[0::2] = rad * np.cos(2 * np.pi * u2)
+    out[1::2] = rad * np.sin(2 * np.pi * u2)
+    return out
+
+
+def taylor_green_noise(shape, u0, sigma, seed, chunk=1 << 21):
+    """Taylor-Green vortex at cell centres x = (i + 1/2), k = 2 pi / n per
+    axis: u = u0 (sin kx cos ky cos kz, -cos kx sin ky cos kz, 0), plus
+    Gaussian noise of standard deviation sigma on every component, drawn in
+    cell-major order (u_x, u_y, u_z of cell 0, then cell 1, ...).  Built in
+    chunks of cells to bound the temporaries.  Returns (rho[n], u[n, 3])."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    u = np.empty((n, 3))
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        i = np.arange(c0, c1)
+        X = (i % nx + 0.5) * 2 * np.pi / nx
+        Y = ((i // nx) % ny + 0.5) * 2 * np.pi / ny
+        Z = (i // (nx * ny) + 0.5) * 2 * np.pi / nz
+        blk = u[c0:c1]
+        blk[:, 0] = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+        blk[:, 1] = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+        blk[:, 2] = 0.0
+        if sigma > 0:
+            # values 3*c0 .. 3*c1-1 of the normal stream; pairs start at even values
+            v0, v1 = 3 * c0, 3 * c1
+            p0, p1 = v0 // 2, (v1 + 1) // 2
+            g = _box_muller(p0, p1 - p0, seed)[v0 - 2 * p0:v1 - 2 * p0]
+            blk += sigma * g.reshape(-1, 3)
+    return np.ones(n), u
 
 WALLS2 = {"-y": ("wall", None, 1.0), "+y": ("wall", None, 1.0)}
 
