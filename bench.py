@@ -3,7 +3,8 @@
 Workload (N=1): 256^3 periodic box, D3Q19 BGK, tau = 0.6, FP32 populations
 (stored shifted, f - w), Taylor-Green vortex u0 = 0.05 plus seeded Gaussian
 noise sigma = 1e-3, rho = 1.  A step is one fused collide+stream launch over
-all 16.8M cells.  W warm-up steps, then K timed steps bracketed by
+all 16.8M cells.  W warm-up steps, untimed settling groups until the step
+time is stable (reported in "settle"), then K timed steps bracketed by
 synchronisation, run without a break as five repetitions with HIP events
 recorded between them on the stream the kernel is launched on; the reported
 time per step is the median over the repetitions (min, max and the mean
@@ -56,6 +57,21 @@ def repetitions(k, r=5):
     r = max(1, min(r, k))
     base, extra = divmod(k, r)
     return [base + (1 if i >= r - extra else 0) for i in range(r)]
+
+
+def settle(lat, group=10, max_groups=30, tol=0.02):
+    """Untimed groups of steps after the warm-up until the step time is
+    stable: three consecutive groups within tol of each other.  On MI355X
+    the first ~10 ms of full HBM load run up to 12 % slower than the
+    sustained rate (a kernel trace shows steps 6-25 of a fresh run at
+    464-489 us against 434 us before and after), so a short timed window
+    right after a 5-step warm-up measures that transient, not the kernel."""
+    hist = []
+    for _ in range(max_groups):
+        hist.append(float(lat.time_step_groups([group])[0]) / group)
+        if len(hist) >= 3 and max(hist[-3:]) <= min(hist[-3:]) * (1 + tol):
+            break
+    return group * len(hist), hist
 
 
 def load_traffic(name):
@@ -204,6 +220,14 @@ def run_warmstart(args):
     def log(msg):
         print(msg, flush=True)
 
+    if os.environ.get("ROCPROF_OUTPUT_PATH"):
+        # Under rocprofv3 --kernel-trace this workload's first lbm_batch_run
+        # ended in a host SIGSEGV inside the HIP runtime (twice; the same
+        # call runs clean without the profiler and the batch workload runs
+        # clean under it).  The cause is not known, so the combination is
+        # refused; profile the network with --workload train and the batched
+        # solver with --workload batch.
+        sys.exit("bench.py --workload warmstart does not run under rocprofv3 (see run_warmstart)")
     t0 = time.perf_counter()
     sur = S.Surrogate(batch=16, height=WS.NY, width=WS.NX, seed=7)
     if args.load_model:
@@ -315,6 +339,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--settle", type=int, default=1,
+                    help="after the warm-up, run untimed groups of 10 steps until the step time is stable")
     ap.add_argument("--workload", default="tgv", choices=["tgv", "batch", "sphere", "warmstart", "train"],
                     help="tgv: the headline 256^3 box (default); batch / sphere / warmstart: time-to-residual "
                          "workloads; train: training-step throughput of the warm-start network")
@@ -366,6 +392,7 @@ def main():
     del rho0, u0
     lat.step(args.warmup)
     lat.sync()
+    settle_steps, settle_ms = settle(lat) if args.settle else (0, [])
     groups = repetitions(args.steps)
     t0 = time.perf_counter()
     group_ms = lat.time_step_groups(groups)  # HIP events on the kernel's stream between repetitions
@@ -403,6 +430,9 @@ def main():
                    "ms_per_step_median": round(ms_per_step, 5), "ms_per_step_min": round(float(per_step.min()), 5),
                    "ms_per_step_max": round(float(per_step.max()), 5),
                    "ms_per_step_mean": round(float(group_ms.sum()) / args.steps, 5)},
+        "settle": {"steps": settle_steps, "ms_per_step_of_groups": [round(x, 4) for x in settle_ms],
+                   "rule": "untimed groups of 10 steps after the warm-up until three in a row agree within 2 % "
+                           "(at most 300 steps)"},
         "wall_s_timed": round(wall, 4),
         "mass_after": s.mass,
     }
