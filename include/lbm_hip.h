@@ -165,6 +165,20 @@ int lbm_step_check(lbm_handle *h, lbm_sums *out);
 #define LBM_STEP_CHECK_SNAPSHOT 3
 int lbm_step_ex(lbm_handle *h, int32_t mode, lbm_sums *out);
 
+/* lbm_step_ex without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The FP64
+ * per-block partial sums and the reduced sums of a check step live in one
+ * library-wide device array per GPU (LBM_NOALLOC_MAX_BLOCKS blocks of 256
+ * cells per launch, about 12.6 MB, held until the process ends) instead of
+ * the handle's buffers; a launch with more blocks is rejected with
+ * LBM_EINVAL.  The snapshot buffers must exist (lbm_init_fields creates
+ * them); the call never creates them.  Results equal lbm_step_ex's bit for
+ * bit.  All calls on one GPU share the array: calls that overlap in time on
+ * one GPU (two host threads, or two handles' streams) race. */
+#define LBM_NOALLOC_MAX_BLOCKS 131072
+int lbm_step_ex_noalloc(lbm_handle *h, int32_t mode, lbm_sums *out);
+
 /* Macroscopic fields after the last step (updateMacroscopicValues,
  * src/lbm/solver.cpp:513-553): rho[n], u[n*dim], cell-major. */
 int lbm_get_macros(lbm_handle *h, double *rho, double *u);

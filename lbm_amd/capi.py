@@ -48,7 +48,8 @@ class Sums(ct.Structure):
 
 
 # every symbol declared in include/lbm_hip.h
-EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_get_macros",
+EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_step_ex_noalloc",
+           "lbm_get_macros",
            "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_time_step_groups", "lbm_set_launch",
            "lbm_set_index_width", "lbm_set_storage", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
@@ -71,6 +72,7 @@ def load():
     lib.lbm_step.argtypes = [P, ct.c_int32]
     lib.lbm_step_check.argtypes = [P, ct.POINTER(Sums)]
     lib.lbm_step_ex.argtypes = [P, ct.c_int32, ct.POINTER(Sums)]
+    lib.lbm_step_ex_noalloc.argtypes = [P, ct.c_int32, ct.POINTER(Sums)]
     lib.lbm_get_macros.argtypes = [P, dp, dp]
     lib.lbm_reduce.argtypes = [P, ct.POINTER(Sums)]
     lib.lbm_get_populations.argtypes = [P, dp]
@@ -175,6 +177,14 @@ class Lattice:
     def step_ex(self, mode):
         s = Sums()
         rc = self.lib.lbm_step_ex(self.h, int(mode), ct.byref(s))
+        if rc not in (LBM_OK, LBM_DIVERGED):
+            self._check(rc)
+        return s, rc
+
+    def step_ex_noalloc(self, mode):
+        """lbm_step_ex through the allocation-free entry point."""
+        s = Sums()
+        rc = self.lib.lbm_step_ex_noalloc(self.h, int(mode), ct.byref(s))
         if rc not in (LBM_OK, LBM_DIVERGED):
             self._check(rc)
         return s, rc

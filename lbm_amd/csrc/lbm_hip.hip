@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -209,6 +210,16 @@ __global__ void __launch_bounds__(256) k_copy(const float4 *__restrict__ in, flo
 
 }  // namespace lbm
 
+// Library-wide scratch of lbm_step_ex_noalloc: the FP64 per-block partial
+// sums of a check step (NSUM arrays of up to LBM_NOALLOC_MAX_BLOCKS, block
+// order) followed by the NSUM reduced sums.  Every check launch writes all
+// partials of its blocks and k_finalize writes all sums, so nothing needs
+// clearing between calls.  Each device holds its own copy until the process
+// ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+constexpr long long NOALLOC_DOUBLES = (long long)LBM_NOALLOC_MAX_BLOCKS * lbm::NSUM + lbm::NSUM;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
 // ============================================================================
 // C ABI
 using namespace lbm;
@@ -237,6 +248,7 @@ struct lbm_handle {
   int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
   bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
   long long steps = 0;
+  int device = 0;  // device current at lbm_create
   std::string err;
 };
 
@@ -301,6 +313,29 @@ static int ensure_io(lbm_handle *h, size_t bytes) {
   h->io_cap = 0;
   HIPCHK(h, hipMalloc(&h->io, bytes));
   h->io_cap = bytes;
+  return LBM_OK;
+}
+
+// Device address of `scratch` on device dev, resolved once per device with
+// that device current (lbm_create resolves it for its device).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int noalloc_scratch(lbm_handle *h, int dev, double **out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(h, LBM_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(h, hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(h, hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCHK(h, hipSetDevice(cur));
+    if (e != hipSuccess) return fail(h, LBM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
   return LBM_OK;
 }
 
@@ -453,6 +488,11 @@ int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out)
   h->by = 256 / h->bx;
   int rc = LBM_OK;
   auto bail = [&](int code) { lbm_destroy(h); return code; };
+  double *scratch_ptr = nullptr;
+  if (hipGetDevice(&h->device) != hipSuccess || noalloc_scratch(nullptr, h->device, &scratch_ptr) != LBM_OK) {
+    fail(nullptr, LBM_EHIP, "lbm_create: device set-up failed: " + g_err);
+    return bail(LBM_EHIP);
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     fail(nullptr, LBM_EHIP, "lbm_create: hipStreamCreate failed");
     return bail(LBM_EHIP);
@@ -617,14 +657,16 @@ static int ensure_snapshots(lbm_handle *h) {
   return LBM_OK;
 }
 
-static int launch_step(lbm_handle *h, int mode) {
+// partials: where a check step puts its per-block sums (the handle's buffer
+// unless given)
+static int launch_step(lbm_handle *h, int mode, double *partials = nullptr) {
   KArgs a = make_args(h);
   if (mode != 0) {
     const int rc = ensure_snapshots(h);
     if (rc) return rc;
     a.snap_in = h->snap[h->snap_last];
     a.snap_out = h->snap[h->snap_last ^ 1];
-    a.partials = h->partials;
+    a.partials = partials ? partials : h->partials;
   }
   dim3 g, b;
   launch_dims(h, g, b);
@@ -691,6 +733,34 @@ int lbm_step_ex(lbm_handle *h, int32_t mode, lbm_sums *out) {
   rc = launch_step(h, mode);
   if (rc) return rc;
   if (mode >= 2) return finalize_sums(h, out);
+  return LBM_OK;
+}
+
+int lbm_step_ex_noalloc(lbm_handle *h, int32_t mode, lbm_sums *out) {
+  if (!h || mode < 0 || mode > 3 || (mode >= 2 && !out)) return fail(h, LBM_EINVAL, "lbm_step_ex_noalloc: bad argument");
+  if (mode == 0) return launch_step(h, 0);
+  const long long nb = nblocks_of(h);
+  if (mode >= 2 && nb > LBM_NOALLOC_MAX_BLOCKS)
+    return fail(h, LBM_EINVAL, "lbm_step_ex_noalloc: the step launch has " + std::to_string(nb) +
+                                   " blocks, above LBM_NOALLOC_MAX_BLOCKS (" + std::to_string(LBM_NOALLOC_MAX_BLOCKS) +
+                                   "); use lbm_step_ex");
+  if (!h->snap[0] || !h->snap[1])
+    return fail(h, LBM_EINVAL, "lbm_step_ex_noalloc: no snapshot buffers yet (lbm_init_fields creates them)");
+  if (mode == 1) return launch_step(h, 1);
+  double *scr = nullptr;
+  int rc = noalloc_scratch(h, h->device, &scr);
+  if (rc) return rc;
+  // the partials of this launch (NSUM arrays of nb blocks), then the sums
+  double *partials = scr, *sums = scr + (long long)LBM_NOALLOC_MAX_BLOCKS * NSUM;
+  rc = launch_step(h, mode, partials);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_finalize, dim3(NSUM), dim3(1024), 0, h->stream, (const double *)partials, nb, sums);
+  HIPCHK(h, hipGetLastError());
+  double s[NSUM];
+  HIPCHK(h, hipMemcpyAsync(s, sums, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  fill_sums(s, h->dim, out);
+  if (s[10] > 0) return fail(h, LBM_DIVERGED, "non-finite values (solution diverged)");
   return LBM_OK;
 }
 
