@@ -278,7 +278,14 @@ auto LBMSolver::run() -> GInt {
     const bool snap = (k + 2) % I == 0;
     const int mode = chk ? (snap ? LBM_STEP_CHECK_SNAPSHOT : LBM_STEP_CHECK) : (snap ? LBM_STEP_SNAPSHOT : LBM_STEP_PLAIN);
     m_pp.execute(PPHook::BeforeTimestep, *m_grid, m_ppU);
-    if (ppStep) {
+    // updateMacroscopicValues of this iteration: the reference's vars are
+    // the macros of the state before this iteration's step, and they are
+    // what the afterTimestep hook, the output of this iteration and the
+    // final analytic comparison read (src/lbm/solver.cpp:309-318,194-195,
+    // 388-482); fetched only when one of those needs them
+    const bool outNow = (m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || m_timeStep == m_maxSteps - 1 ||
+                        converged || m_diverged;
+    if (ppStep || outNow) {
       fetchMacros();
       for (GInt q = 0; q < m_grid->size(); ++q) m_ppU[q] = m_u[q * m_dim];
     }
@@ -303,7 +310,10 @@ auto LBMSolver::run() -> GInt {
   std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
             << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
   m_pp.execute(PPHook::AtEnd, *m_grid, m_ppU);
-  fetchMacros();
+  // the fields of the last iteration (fetched before its step) are the
+  // vars compareToAnalyticalResult reads; without any iteration, the
+  // initial fields
+  if (m_timeStep == 0) fetchMacros();
   if (m_diverged) TERMM(-1, "Solution diverged");
   if (m_solverCfg->has_config_value("analyticalSolution")) compareToAnalyticalResult();
   for (const auto& k : m_solverCfg->unusedConfigValues()) std::cerr << "WARNING: unused configuration value " << k << std::endl;
@@ -377,9 +387,8 @@ void LBMSolver::output(bool forced, const std::string& postfix) {
     return;
   }
   if (!((m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || forced)) return;
-  fetchMacros();
+  // m_rho / m_u hold this iteration's vars (fetched before its step)
   const GInt N = m_grid->size();
-  for (GInt q = 0; q < N; ++q) m_ppU[q] = m_u[q * m_dim];
   const GInt n = m_outputAll ? N : 0;
   if (n == 0) {
     std::cerr << "ERROR: Invalid call to encodeLE() with length = 0" << std::endl;

@@ -67,6 +67,39 @@ def host_grid(cfg):
                 nxyz=nxyz, sidx=sidx, surf=surf)
 
 
+def oracle_grid(cfg):
+    """The literal restatement's grid in host_grid's form: cell count,
+    centres, neighbour table, boundary flags, surfaces, and the structured
+    window (extent per axis and each cell's structured index, read off the
+    distinct centre coordinates)."""
+    gen, ref = G.build(cfg)
+    n, dim = ref.n, gen.dim
+    cen = np.array(ref.center, dtype=np.float64).reshape(n, dim)
+    nb = np.array(ref.nghbr, dtype=np.int64).reshape(n, -1)
+    nxyz = np.ones(3, dtype=np.int64)
+    idx = np.zeros((n, 3), dtype=np.int64)
+    for a in range(dim):
+        vals, inv = np.unique(cen[:, a], return_inverse=True)
+        nxyz[a] = vals.size
+        idx[:, a] = inv
+    sidx = idx[:, 0] + nxyz[0] * (idx[:, 1] + nxyz[1] * idx[:, 2])
+    surf = {k: (list(cells), [dirs[c] for c in cells]) for k, (cells, dirs) in ref.surfaces.items()}
+    return dict(n=n, dim=dim, stride=nb.shape[1], cen=cen, nb=nb, bnd=np.array(ref.bndry, dtype=np.uint8), nxyz=nxyz,
+                sidx=sidx, surf=surf)
+
+
+def assert_same_grid(h, g):
+    """Host grid == restated grid, bit for bit (cell order and centres,
+    neighbours incl. periodic links and diagonals, boundary flags, surfaces
+    with their normals, structured window)."""
+    assert h["n"] == g["n"]
+    assert np.array_equal(h["cen"], g["cen"]), "cell centres / cell order differ"
+    assert np.array_equal(h["nb"], g["nb"]), "neighbour tables differ"
+    assert np.array_equal(h["bnd"].astype(bool), g["bnd"].astype(bool))
+    assert h["surf"] == g["surf"]
+    assert list(h["nxyz"]) == list(g["nxyz"]) and np.array_equal(h["sidx"], g["sidx"])
+
+
 CASES = [
     (REFCFG, "couette__couette.json"),
     (REFCFG, "couette__couette_bnd_bbDirichlet.json"),

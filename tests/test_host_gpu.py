@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.test_grid import host_grid
+from tests.test_grid import assert_same_grid, host_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -103,37 +103,87 @@ def test_cli_line_postprocessing(tmp_path):
     assert len(text.splitlines()) == len(cells) + 1 and len(cells) >= 32
 
 
-WETNODE_REFCFG = ["couette__couette_bnd_eq.json", "couette__couette_bnd_eq2.json", "couette__couette_bnd_eq_aligned.json",
-                  "couette__couette_bnd_NEEM.json", "couette__couette_bnd_NEBB.json", "poiseuille__poiseuille_bnd_eq.json",
-                  "poiseuille__poiseuille_bnd_NEEM.json", "poiseuille__poiseuille_bnd_NEBB.json",
-                  "poiseuille__poiseuille_bnd_pressure.json", "poiseuille__poiseuille_bnd_pressure_neem2.json"]
+POISEUILLE_WETNODE_REFCFG = ["poiseuille__poiseuille_bnd_eq.json", "poiseuille__poiseuille_bnd_NEEM.json",
+                             "poiseuille__poiseuille_bnd_NEBB.json", "poiseuille__poiseuille_bnd_pressure.json",
+                             "poiseuille__poiseuille_bnd_pressure_neem2.json"]
 
 
-@pytest.mark.parametrize("name", WETNODE_REFCFG)
-def test_wetnode_reference_thresholds_f64(name, tmp_path):
-    """The reference's Couette and Poiseuille cases with wet-node walls
-    (equilibrium, NEEM, NEBB) through the CLI in FP64: each must pass its own
-    analytic error thresholds (exit status 0).
-
-    The Couette cases converge to an exactly linear profile, and their
-    thresholds sit at the level the run that set them reached: stopped by
-    their own criterion (one-step change < 5e-13 .. 1e-12) at step 14,801,
-    this build is 0.4 % above errorMax (6.345e-13 vs 6.31e-13 / 6.32e-13)
-    with errorL2 3.0e-13 (1e-13 asked).  They run here with the criterion
-    at 1e-15, where the remaining error is rounding; thresholds below 1e-13
-    (couette_bnd_NEEM: 3.46e-15 and 1e-15) are checked at 1e-13."""
+@pytest.mark.parametrize("name", POISEUILLE_WETNODE_REFCFG)
+def test_poiseuille_wetnode_reference_thresholds_f64(name, tmp_path):
+    """The reference's Poiseuille cases with wet-node walls and pressure
+    faces, unmodified, through the CLI in FP64: each passes its own analytic
+    error thresholds (exit status 0)."""
     cfg = json.load(open(os.path.join(REFCFG, name)))
-    if name.startswith("couette"):
-        s = cfg["solver"]
-        s["convergence"] = 1e-15
-        s["maxSteps"] = 60000
-        s["errorMax"] = max(s["errorMax"], 1e-13)
-        s["errorL2"] = max(s["errorL2"], 1e-13)
     r = run_cli(cfg, tmp_path, "-p", "f64", timeout=600)
     tail = "\n".join(l for l in r.stderr.splitlines() if "Error" in l or "error" in l or "convergence" in l)
     print(name, r.returncode, tail[-1500:])
     assert r.returncode == 0, r.stderr[-2000:]
     assert "max. Error" in r.stderr
+
+
+def cli_report(stderr):
+    """(last check iteration, errorMax, errorL2) printed by the CLI."""
+    stop, emax, l2 = None, None, None
+    for line in stderr.splitlines():
+        head = line.split(":")[0]
+        if head.isdigit() and " dU=" in line:
+            stop = int(head)
+        if line.startswith("max. Error:"):
+            emax = float(line.split(":")[1])
+        if line.startswith("avg. L2:"):
+            l2 = float(line.split(":")[1])
+    return stop, emax, l2
+
+
+@pytest.mark.parametrize("name", ["couette__couette_bnd_eq.json", "couette__couette_bnd_eq2.json",
+                                  "couette__couette_bnd_eq_aligned.json", "couette__couette_bnd_NEEM.json",
+                                  "couette__couette_bnd_NEBB.json"])
+def test_couette_wetnode_reference_cases_f64(name, tmp_path):
+    """The reference's Couette cases with wet-node walls, unmodified, through
+    the CLI in FP64, against the oracle run under the reference's own loop
+    and stopping rule on the restated grid (tests/test_wetnode.py,
+    reference_stop_rule): the check that stops the run comes at the same
+    iteration, errorMax and errorL2 agree to the rounding noise of these
+    15,000-step FP64 runs, and the run fails its thresholds as the
+    restatement does.
+
+    The errors measured here are the remaining transient, ~1e-12 on
+    velocities of ~0.05, i.e. differences at the 1e-11 relative level; the
+    oracle's unshifted FP64 populations (~0.1-0.4) carry ~1e-14 of
+    accumulated rounding in u after 15,000 steps, so errorMax agrees to
+    5e-14 absolute (a few % of its value) and errorL2 to 5 %.  With the
+    reference's L2 formula the error profile, the slowest decaying mode,
+    gives errorL2 about 0.48 x errorMax: 3.0e-13 where 1e-13 is asked (and
+    6.7e-13 for couette_bnd_eq), so those four cases end with exit status
+    255 through 'Analytical testcase failed', as the reference's own
+    algorithm does on these inputs.  couette_bnd_NEEM ends on the exact
+    profile up to rounding (3.46e-15 / 1.6e-15 in the restatement, which
+    never meets its 1e-14 criterion and runs to maxSteps)."""
+    from tests.test_wetnode import analytic_errors, reference_stop_rule
+
+    cfg = json.load(open(os.path.join(REFCFG, name)))
+    s = cfg["solver"]
+    stop, (rho_o, u_o), g = reference_stop_rule(cfg)
+    emax_o, l2_o = analytic_errors(cfg, g["cen"], u_o)
+    r = run_cli(cfg, tmp_path, "-p", "f64", timeout=600)
+    cstop, emax, l2 = cli_report(r.stderr)
+    print(f"{name}: restated stop {stop} errorMax {emax_o:.6e} errorL2 {l2_o:.6e}; "
+          f"CLI exit {r.returncode} last check {cstop} errorMax {emax} errorL2 {l2}")
+    if emax_o < 1e-13:
+        # couette_bnd_NEEM reaches the exact profile up to rounding.  Its
+        # criterion, 1e-14 for sum |du| over 640 cells, sits at the rounding
+        # floor: the unshifted FP64 restatement (like the reference) never
+        # meets it and runs to maxSteps, the shifted FP64 device populations
+        # round less and may meet it earlier; either way both errors are
+        # rounding noise, and passing 3.46e-15 / 1e-15 is decided by it
+        assert emax < 1e-13 and l2 < 1e-13
+        return
+    assert "Reached convergence" in r.stderr and cstop == stop
+    assert abs(emax - emax_o) <= 5e-14 and abs(l2 - l2_o) <= 0.05 * l2_o
+    # errorL2 fails by a factor near 3 in both: far outside the rounding noise
+    assert l2_o > 2 * s["errorL2"] and l2 > 2 * s["errorL2"]
+    assert r.returncode == 255, r.stderr[-2000:]
+    assert "Analytical testcase failed" in r.stderr
 
 
 def test_couette_reference_f32_fails_loudly_only_on_thresholds(tmp_path):
@@ -229,45 +279,78 @@ def wrapped_neighbours(h, cfg):
 
 
 CASES = [
-    ("couette__couette.json", REFCFG, 400, False),
-    ("couette__couette_bnd_bbDirichlet.json", REFCFG, 400, False),
-    ("pressure_channel2d.json", OWNCFG, 400, False),
-    ("box3d_periodic_x.json", OWNCFG, 200, False),
-    ("sphere3d_channel.json", OWNCFG, 150, True),
-    ("neem_channel2d.json", OWNCFG, 400, False),
-    ("sphere3d_neem.json", OWNCFG, 150, True),
-    ("poiseuille__poiseuille_bnd.json", REFCFG, 300, False),
-    ("couette__couette_bnd_eq.json", REFCFG, 300, False),
-    ("couette__couette_bnd_eq2.json", REFCFG, 300, False),
-    ("couette__couette_bnd_eq_aligned.json", REFCFG, 300, False),
-    ("couette__couette_bnd_NEEM.json", REFCFG, 300, False),
-    ("couette__couette_bnd_NEBB.json", REFCFG, 300, False),
-    ("poiseuille__poiseuille_bnd_eq.json", REFCFG, 300, False),
-    ("poiseuille__poiseuille_bnd_NEEM.json", REFCFG, 300, False),
-    ("poiseuille__poiseuille_bnd_NEBB.json", REFCFG, 300, False),
-    ("poiseuille__poiseuille_bnd_pressure.json", REFCFG, 300, False),
+    ("couette__couette.json", REFCFG, 400, False, None),
+    ("couette__couette_bnd_bbDirichlet.json", REFCFG, 400, False, None),
+    ("pressure_channel2d.json", OWNCFG, 400, False, None),
+    ("box3d_periodic_x.json", OWNCFG, 200, False, None),
+    ("sphere3d_channel.json", OWNCFG, 150, True, None),
+    ("neem_channel2d.json", OWNCFG, 400, False, None),
+    ("sphere3d_neem.json", OWNCFG, 150, True, None),
+    ("poiseuille__poiseuille_bnd.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd.json", REFCFG, 300, False, 1.002),
+    ("couette__couette_bnd_eq.json", REFCFG, 300, False, None),
+    ("couette__couette_bnd_eq2.json", REFCFG, 300, False, None),
+    ("couette__couette_bnd_eq_aligned.json", REFCFG, 300, False, None),
+    ("couette__couette_bnd_NEEM.json", REFCFG, 300, False, None),
+    ("couette__couette_bnd_NEBB.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd_eq.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd_eq.json", REFCFG, 300, False, 1.002),
+    ("poiseuille__poiseuille_bnd_NEEM.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd_NEEM.json", REFCFG, 300, False, 1.002),
+    ("poiseuille__poiseuille_bnd_NEBB.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd_NEBB.json", REFCFG, 300, False, 1.002),
+    ("poiseuille__poiseuille_bnd_pressure.json", REFCFG, 300, False, None),
+    ("poiseuille__poiseuille_bnd_pressure.json", REFCFG, 300, False, 1.002),
 ]
 
 
-@pytest.mark.parametrize("name,folder,steps,textbook", CASES)
+_GRIDS = {}
+
+
+def restated_grid(folder, name):
+    from tests.test_grid import oracle_grid
+
+    if (folder, name) not in _GRIDS:
+        _GRIDS[folder, name] = oracle_grid(json.load(open(os.path.join(folder, name))))
+    return _GRIDS[folder, name]
+
+
+@pytest.mark.parametrize("name,folder,steps,textbook,jump", CASES)
 @pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 1e-5)])
-def test_host_matches_oracle_on_reference_grid(name, folder, steps, textbook, storage, tol, tmp_path):
+def test_host_matches_oracle_on_reference_grid(name, folder, steps, textbook, jump, storage, tol, tmp_path):
+    """The host on the reference grid against the oracle stepped on the
+    restated grid with the reference's boundary semantics, maxSteps
+    iterations with the thresholds and the stop criterion removed.
+
+    The reference's Poiseuille cases drive velocities of ~1e-7 with density
+    jumps of ~1e-6 between their pressure faces; there the oracle's
+    unshifted FP64 populations (~0.1, rounding ~1e-17 per operation) hold u
+    to ~1e-9 relative per step, so the FP64 bar of those unmodified cases is
+    1e-8 after their 300 steps.  The same cases with a 2e-3 jump
+    (velocities ~2e-4, the jump column) are checked at the 1e-11 bar."""
     cfg = json.load(open(os.path.join(folder, name)))
+    jumps = [abs(v.get("pressure", 1.0) - 1.0) for faces in cfg["solver"]["boundary"].values() for v in faces.values()]
+    if storage == "f64" and jump is None and 0 < max(jumps) < 1e-4:
+        tol = 1e-8
     s = cfg["solver"]
     s["maxSteps"] = steps
     s["convergence"] = 0.0
     for k in ("analyticalSolution", "errorL2", "errorMax", "errorGRE", "analyticalSolutionExcludeSurface",
               "postprocessing", "poiseuillePressureGradient"):
         s.pop(k, None)
-    for spec in s["boundary"].get("cube", {}).values():
-        if spec.get("pressure", 1.0) != 1.0:
-            # a 2e-3 density jump instead of 8e-7: at velocities of ~1e-7
-            # the oracle's unshifted FP64 populations (~0.1) leave only
-            # ~1e-10 relative precision in u; the semantics are the same
-            spec["pressure"] = 1.002
+    if jump is not None:
+        for spec in s["boundary"].get("cube", {}).values():
+            if spec.get("pressure", 1.0) != 1.0:
+                spec["pressure"] = jump
     h, rho, u, done = host_run(cfg, tmp_path, storage)
     assert done == steps
-    r_rho, r_u = oracle_run(cfg, h, steps, textbook)
+    # grid indexing and neighbour maps: the host's grid is the restated
+    # generator's, bit for bit; the oracle steps the restated grid
+    g = restated_grid(folder, name)
+    assert_same_grid(h, g)
+    # the host returns the vars of its last iteration: the macros after
+    # maxSteps - 1 steps
+    r_rho, r_u = oracle_run(cfg, g, steps - 1, textbook)
     eu = np.linalg.norm(u - r_u) / np.linalg.norm(r_u)
     er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
     print(f"{name} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
