@@ -32,8 +32,9 @@ extern "C" {
 #define LBM_D2Q9 9
 #define LBM_D3Q19 19
 
-/* population storage.  Every form stores f_i - w_i ("shifted"); collision
- * runs in FP32 for F32/F16/BF16 and in FP64 for F64. */
+/* population storage.  Every form stores f_i - w_i ("shifted"); moments,
+ * equilibrium and collision run in FP64 for F32 and F64 storage and in FP32
+ * for F16/BF16 storage. */
 #define LBM_STORE_F32 0
 #define LBM_STORE_F64 1
 #define LBM_STORE_F16 2
@@ -211,6 +212,17 @@ int lbm_time_step_groups(lbm_handle *h, const int32_t *steps, int32_t ngroups, d
  * block_y rows (0 keeps the current value); nontemporal loads/stores on (1)
  * or off (0), -1 keeps the current setting. */
 int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nontemporal);
+
+/* How the step kernel gathers the populations with c_x != 0: 0 (default)
+ * straight from global memory, neighbours' cache lines reused through L2;
+ * 1 through an LDS stage per 256-cell x row (aligned row loads plus two
+ * halo cells per row and population, then shifted reads from LDS).  Same
+ * results bit for bit.  1 needs FP32 storage, 32-bit offsets and 256x1
+ * blocks (x extent >= 256); it is dropped again when lbm_set_storage
+ * leaves FP32, and lbm_set_launch / lbm_set_index_width shapes other than
+ * those fall back to 0.  Replaces the push of propagationStep,
+ * src/lbm/solver.cpp:715-740. */
+int lbm_set_halo(lbm_handle *h, int32_t lds);
 
 /* Width of the population offsets of the step kernel: 32 (default whenever a
  * population plane fits in 4 GiB; one 32-bit offset per access) or 64.

@@ -618,6 +618,28 @@ __device__ __forceinline__ void wet_wall(const G &a, int x, int y, int z, int f,
   }
 }
 
+// Index of population j among the populations with c_x != 0 (the ones an
+// x-row tile pulls from a shifted position), in population order; -1 for
+// c_x == 0.  NXS: how many there are (6 for D2Q9, 10 for D3Q19).
+template <int Q>
+__host__ __device__ constexpr int xshift_index(int j) {
+  int k = 0;
+  for (int i = 0; i < j; ++i) k += Lat<Q>::cx[i] != 0;
+  return Lat<Q>::cx[j] != 0 ? k : -1;
+}
+template <int Q>
+__host__ __device__ constexpr int count_xshift() {
+  int k = 0;
+  for (int i = 0; i < Q; ++i) k += Lat<Q>::cx[i] != 0;
+  return k;
+}
+template <int Q>
+constexpr int NXS = count_xshift<Q>();
+static_assert(NXS<9> == 6 && NXS<19> == 10, "x-shifted populations per lattice");
+
+// Row width of the LDS x-halo stage: a 256-cell row plus one cell each side.
+constexpr int LDS_ROW = 256 + 2;
+
 // --------------------------------------------------------------------------
 // The fused step.  MODE 0: plain; 1: also write (u, rho) snapshot;
 // 2: reduce against the snapshot written by the previous step; 3: both
@@ -632,8 +654,14 @@ __device__ __forceinline__ void wet_wall(const G &a, int x, int y, int z, int f,
 // solid: TOPO 1, every neighbour is a fluid cell inside the domain (no
 // wrap); TOPO 2, every axis is periodic and there are no solids (wraps
 // only).
-template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO = 0, typename G>
-__device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, double (&red)[NSUM]) {
+// LDSX: the populations with c_x != 0 come from the block's LDS stage xs
+// (k_step_lds: row of plane j at the source (y, z), cells x0 - 1 .. x0 + 256
+// in slots 0 .. 257) instead of shifted global loads, and those with
+// c_x == 0 from pre[j], loaded by the caller.
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO = 0, bool LDSX = false, typename G>
+__device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, double (&red)[NSUM],
+                                            const typename Store<ST>::R *xs = nullptr,
+                                            const typename Store<ST>::R *pre = nullptr) {
   using L = Lat<Q>;
   using C = typename Store<ST>::C;
   using I = Idx<W64>;
@@ -686,12 +714,21 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
       bool valid = (cx > 0 ? vxm : (cx < 0 ? vxp : true)) && (cy > 0 ? vym : (cy < 0 ? vyp : true)) &&
                    (cz > 0 ? vzm : (cz < 0 ? vzp : true));
       const I s = I(sx) + I(a.px) * (I(sy) + I(a.ny) * I(sz));
-      // a missing upstream cell (across a face) loads the cell's own entry
-      // of plane j (in bounds, same cache lines as the neighbours' loads)
-      // and is replaced below, so every load of the loop keeps the uniform
-      // base of plane j
-      const ST *p = pop_ptr<ST, W64>(src, plane, j, valid ? s : cell);
-      h[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
+      if (LDSX && cx != 0) {
+        // staged: slot (x - x0) + 1 - c_x of the row of plane j
+        h[j] = xs[xshift_index<Q>(j) * LDS_ROW + (int)threadIdx.x + 1 - cx];
+      } else if (LDSX) {
+        // c_x == 0: loaded by the caller before the stage's barrier (the
+        // same source, or the cell's own entry when the source is missing)
+        h[j] = pre[j];
+      } else {
+        // a missing upstream cell (across a face) loads the cell's own entry
+        // of plane j (in bounds, same cache lines as the neighbours' loads)
+        // and is replaced below, so every load of the loop keeps the uniform
+        // base of plane j
+        const ST *p = pop_ptr<ST, W64>(src, plane, j, valid ? s : cell);
+        h[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
+      }
       if (!valid) missing |= 1u << j;
     }
     // A population streams iff its upstream cell is fluid.  This is the

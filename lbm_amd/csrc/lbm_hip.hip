@@ -247,6 +247,7 @@ struct lbm_handle {
   hipStream_t stream = nullptr;
   int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
   bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
+  bool lds = false;              // LDS-staged x halo (lbm_set_halo)
   long long steps = 0;
   int device = 0;  // device current at lbm_create
   std::string err;
@@ -672,7 +673,8 @@ static int launch_step(lbm_handle *h, int mode, double *partials = nullptr) {
   launch_dims(h, g, b);
   // a periodic box without solids runs a kernel without boundary paths
   const bool periodic_box = !h->has_flags && h->per[0] && h->per[1] && (h->dim == 2 || h->per[2]);
-  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0, h->w64, periodic_box);
+  dispatch_qs<StepL>(h->Q, h->storage, a, g, b, h->stream, h->has_flags, mode, h->nt != 0, h->w64, periodic_box,
+                     h->lds && h->bx == 256 && h->by == 1);
   HIPCHK(h, hipGetLastError());
   if (mode == 1 || mode == 3) h->snap_last ^= 1;
   h->cur ^= 1;
@@ -869,6 +871,7 @@ int lbm_set_storage(lbm_handle *h, int32_t storage) {
   h->esize = es;
   h->w64 = (unsigned long long)h->plane * (unsigned long long)es > 0xffffffffull;
   h->snap_last = 0;
+  if (storage != LBM_STORE_F32) h->lds = false;
   return LBM_OK;
 }
 
@@ -918,6 +921,14 @@ int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nont
   h->by = by;
   if (nontemporal >= 0) h->nt = nontemporal;
   return ensure_partials(h);
+}
+
+int lbm_set_halo(lbm_handle *h, int32_t lds) {
+  if (!h || (lds != 0 && lds != 1)) return fail(h, LBM_EINVAL, "lbm_set_halo: lds must be 0 or 1");
+  if (lds && (h->storage != LBM_STORE_F32 || h->w64 || h->bx != 256 || h->by != 1))
+    return fail(h, LBM_EINVAL, "lbm_set_halo: the LDS x halo needs FP32 storage, 32-bit offsets and 256x1 blocks");
+  h->lds = lds != 0;
+  return LBM_OK;
 }
 
 int lbm_set_index_width(lbm_handle *h, int32_t bits) {

@@ -57,8 +57,9 @@ template <int Q, typename ST>
 struct StepL {
   // mode: 0 plain, 1 snapshot, 2 check, 3 check + snapshot; nt: nontemporal
   // loads/stores of the plain step; w64: 64-bit population offsets
+  // lds: the LDS-staged x-halo kernel (FP32 storage, 32-bit offsets, 256x1 blocks)
   static void run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64,
-                  bool periodic_box);
+                  bool periodic_box, bool lds);
 };
 
 template <int Q, typename ST>

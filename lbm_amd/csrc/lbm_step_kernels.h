@@ -48,6 +48,138 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWa
   }
 }
 
+// LDS-staged x halo.  The block is one 256-cell x row (blockDim 256 x 1).
+// For every population with c_x != 0 the block first loads the source row of
+// plane j into LDS -- cells x0 .. x0 + 255 one aligned element per lane, and
+// the two halo cells x0 - 1 and x0 + 256 (wrapped on a periodic x axis) --
+// then each cell reads its shifted value there.  Every population element
+// is still read once from memory; the ten (D3Q19) shifted loads per cell,
+// each touching three cache lines per wave, become aligned loads of two
+// lines plus two single-element halo loads per row and plane.
+template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<Q, ST, TOPO>::value)))
+k_step_lds(const KArgs a) {
+  using L = Lat<Q>;
+  using R = typename Store<ST>::R;
+  using I = Idx<W64>;
+  __shared__ R xs[NXS<Q> * LDS_ROW];
+  const int t = threadIdx.x;
+  const int x0 = blockIdx.x * 256, x = x0 + t;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const ST *src = static_cast<const ST *>(a.src);
+  const long long plane = a.plane;
+  const int last = min(x0 + 256, a.nx) - 1;  // last cell of this row
+  const bool perx = TOPO == 2 || a.per[0];
+  // Source row (y - c_y, z - c_z) of every staged population; a row across
+  // a non-periodic face is not staged (the populations it would give are
+  // replaced by the boundary rules) -- its lanes load their own row instead,
+  // so that every load is issued, unconditionally and all before any LDS
+  // store, as in the direct kernel.
+  I rb[NXS<Q>];
+  bool ok[NXS<Q>];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+    if (cx == 0) continue;
+    const int k = xshift_index<Q>(j);
+    int sy = y - cy, sz = z - cz;
+    ok[k] = true;
+    if (sy < 0 || sy >= a.ny) {
+      if (TOPO == 2 || a.per[1]) sy = sy < 0 ? sy + a.ny : sy - a.ny;
+      else { ok[k] = false; sy = y; }
+    }
+    if (sz < 0 || sz >= a.nz) {
+      if (TOPO == 2 || a.per[2]) sz = sz < 0 ? sz + a.nz : sz - a.nz;
+      else { ok[k] = false; sz = z; }
+    }
+    rb[k] = I(a.px) * (I(sy) + I(a.ny) * I(sz));
+  }
+  // the row itself: lane t holds cell x0 + t (lanes past the last cell load
+  // the last cell; their slots are never read)
+  const int xc = min(x, last);
+  R v[NXS<Q>];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    if (L::cx[j] == 0) continue;
+    const int k = xshift_index<Q>(j);
+    const ST *p = pop_ptr<ST, W64>(src, plane, j, rb[k] + I(xc));
+    v[k] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
+  }
+  // the halo cells: lane k < NXS of the last wave loads the left halo of
+  // staged population k, lane 32 + k the right one (wrapped on a periodic
+  // x axis, else absent)
+  R hv = 0;
+  int hslot = -1;
+  const int hl = t - (256 - 64), hk = hl & 31;
+  if (hl >= 0 && hk < NXS<Q>) {
+    int xh = hl < 32 ? x0 - 1 : last + 1;
+    if (xh < 0) xh = perx ? a.nx - 1 : -1;
+    if (xh >= a.nx) xh = perx ? 0 : -1;
+    if (xh >= 0) {
+      int jj = 0;  // population of staged index hk
+#pragma unroll
+      for (int j = 0; j < Q; ++j)
+        if (xshift_index<Q>(j) == hk) jj = j;
+      I r = 0;
+#pragma unroll
+      for (int k = 0; k < NXS<Q>; ++k)
+        if (k == hk) r = rb[k];
+      hv = Store<ST>::ld(pop_ptr<ST, W64>(src, plane, jj, r + I(xh)));
+      hslot = hk * LDS_ROW + (hl < 32 ? 0 : last - x0 + 2);
+    }
+  }
+  // the populations with c_x == 0: straight loads from the (y - c_y, z - c_z)
+  // row, issued before the barrier with the stage's loads so that all Q
+  // loads of a cell are in flight together; a source across a non-periodic
+  // face loads the cell's own entry (replaced by the boundary rules)
+  R pre[Q];
+  if (x <= last) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const int cx = L::cx[j], cy = L::cy[j], cz = L::cz[j];
+      if (cx != 0) continue;
+      int sy = y - cy, sz = z - cz;
+      bool valid = true;
+      if (sy < 0 || sy >= a.ny) {
+        if (TOPO == 2 || a.per[1]) sy = sy < 0 ? sy + a.ny : sy - a.ny;
+        else valid = false;
+      }
+      if (sz < 0 || sz >= a.nz) {
+        if (TOPO == 2 || a.per[2]) sz = sz < 0 ? sz + a.nz : sz - a.nz;
+        else valid = false;
+      }
+      const I src_cell = valid ? I(x) + I(a.px) * (I(sy) + I(a.ny) * I(sz)) : I(x) + I(a.px) * (I(y) + I(a.ny) * I(z));
+      const ST *p = pop_ptr<ST, W64>(src, plane, j, src_cell);
+      pre[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NXS<Q>; ++k)
+    if (ok[k]) xs[k * LDS_ROW + t + 1] = v[k];
+  if (hslot >= 0) xs[hslot] = hv;
+  __syncthreads();
+  double red[NSUM];
+#pragma unroll
+  for (int k = 0; k < NSUM; ++k) red[k] = 0;
+  cell_update<Q, ST, FLAGS, MODE, NT, W64, TOPO, true>(a, x, y, z, red, xs, pre);
+  if (MODE >= 2) {
+    const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
+    const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    block_reduce_store<256>(red, a.partials, b, nb);
+  }
+}
+
+template <int Q, typename ST, bool FLAGS, bool W64, int TOPO>
+static void launch_step_lds(const KArgs &a, dim3 g, hipStream_t s, int mode, bool nt) {
+  const dim3 b(256, 1, 1);
+  if (mode == 0) {
+    if (nt) hipLaunchKernelGGL((k_step_lds<Q, ST, FLAGS, 0, true, W64, TOPO>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_step_lds<Q, ST, FLAGS, 0, false, W64, TOPO>), g, b, 0, s, a);
+  } else if (mode == 1) hipLaunchKernelGGL((k_step_lds<Q, ST, FLAGS, 1, false, W64, TOPO>), g, b, 0, s, a);
+  else if (mode == 2) hipLaunchKernelGGL((k_step_lds<Q, ST, FLAGS, 2, false, W64, TOPO>), g, b, 0, s, a);
+  else hipLaunchKernelGGL((k_step_lds<Q, ST, FLAGS, 3, false, W64, TOPO>), g, b, 0, s, a);
+}
+
 template <int Q, typename ST, bool FLAGS, bool W64, int TOPO>
 static void launch_step_kernel(const KArgs &a, dim3 g, dim3 b, hipStream_t s, int mode, bool nt) {
   if (mode == 0) {
@@ -62,7 +194,15 @@ static void launch_step_kernel(const KArgs &a, dim3 g, dim3 b, hipStream_t s, in
 // FP16 (71.3 vs 65.0 GLUPS) and 3.6 % slower for FP32 (37.3 vs 38.7).
 template <int Q, typename ST>
 void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags, int mode, bool nt, bool w64,
-                       bool periodic_box) {
+                       bool periodic_box, bool lds) {
+  if constexpr (std::is_same<ST, float>::value) {
+    // LDS x halo: FP32 storage, 32-bit offsets, 256-cell rows (checked by the host)
+    if (lds && !w64) {
+      if (flags) launch_step_lds<Q, ST, true, false, 0>(a, g, s, mode, nt);
+      else launch_step_lds<Q, ST, false, false, 0>(a, g, s, mode, nt);
+      return;
+    }
+  }
   if constexpr (sizeof(ST) == 2) {
     if (periodic_box && !flags) {
       if (w64) launch_step_kernel<Q, ST, false, true, 2>(a, g, b, s, mode, nt);

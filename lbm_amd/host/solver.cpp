@@ -19,6 +19,8 @@ auto storageFromString(const std::string& s) -> int {
   TERMM(-1, "Invalid storage '" + s + "' (f32, f64, f16, bf16)");
 }
 
+}  // namespace
+
 void mkdirs(const std::string& path) {
   std::string cur;
   for (size_t i = 0; i < path.size(); ++i) {
@@ -26,7 +28,6 @@ void mkdirs(const std::string& path) {
     if (path[i] == '/' || i + 1 == path.size()) mkdir(cur.c_str(), 0755);
   }
 }
-}  // namespace
 
 // ------------------------------------------------------------- grid generator
 void GridGenerator::init(int, char**, std::string config_file) { m_config = Configuration::fromFile(config_file); }
@@ -278,14 +279,7 @@ auto LBMSolver::run() -> GInt {
     const bool snap = (k + 2) % I == 0;
     const int mode = chk ? (snap ? LBM_STEP_CHECK_SNAPSHOT : LBM_STEP_CHECK) : (snap ? LBM_STEP_SNAPSHOT : LBM_STEP_PLAIN);
     m_pp.execute(PPHook::BeforeTimestep, *m_grid, m_ppU);
-    // updateMacroscopicValues of this iteration: the reference's vars are
-    // the macros of the state before this iteration's step, and they are
-    // what the afterTimestep hook, the output of this iteration and the
-    // final analytic comparison read (src/lbm/solver.cpp:309-318,194-195,
-    // 388-482); fetched only when one of those needs them
-    const bool outNow = (m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || m_timeStep == m_maxSteps - 1 ||
-                        converged || m_diverged;
-    if (ppStep || outNow) {
+    if (ppStep) {
       fetchMacros();
       for (GInt q = 0; q < m_grid->size(); ++q) m_ppU[q] = m_u[q * m_dim];
     }
@@ -310,10 +304,7 @@ auto LBMSolver::run() -> GInt {
   std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
             << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
   m_pp.execute(PPHook::AtEnd, *m_grid, m_ppU);
-  // the fields of the last iteration (fetched before its step) are the
-  // vars compareToAnalyticalResult reads; without any iteration, the
-  // initial fields
-  if (m_timeStep == 0) fetchMacros();
+  fetchMacros();
   if (m_diverged) TERMM(-1, "Solution diverged");
   if (m_solverCfg->has_config_value("analyticalSolution")) compareToAnalyticalResult();
   for (const auto& k : m_solverCfg->unusedConfigValues()) std::cerr << "WARNING: unused configuration value " << k << std::endl;
@@ -387,8 +378,13 @@ void LBMSolver::output(bool forced, const std::string& postfix) {
     return;
   }
   if (!((m_timeStep > 0 && m_timeStep % m_solutionInterval == 0) || forced)) return;
-  // m_rho / m_u hold this iteration's vars (fetched before its step)
+  // the reference's output first runs updateMacroscopicValues on the fold
+  // after this iteration's step (src/lbm/solver.cpp:336-337), so the files,
+  // the vars the atEnd hook sees and those compareToAnalyticalResult reads
+  // after the final forced output are the macros after the step
+  fetchMacros();
   const GInt N = m_grid->size();
+  for (GInt q = 0; q < N; ++q) m_ppU[q] = m_u[q * m_dim];
   const GInt n = m_outputAll ? N : 0;
   if (n == 0) {
     std::cerr << "ERROR: Invalid call to encodeLE() with length = 0" << std::endl;

@@ -36,6 +36,9 @@ class GridGenerator : public Runnable {
   std::unique_ptr<CartesianGrid> m_grid;
 };
 
+// create every directory of path (output directories)
+void mkdirs(const std::string& path);
+
 struct SolverOptions {
   std::string storage = "";  // overrides the 'storage' solver key when set
   bool quiet = false;

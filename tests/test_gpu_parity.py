@@ -438,3 +438,86 @@ def test_wet_node_wall_validation():
     lat.set_faces({"-x": ("pressure_abb", None, 1.001), "+x": ("pressure_abb", None, 1.0),
                    "-y": ("wall_neem", None, 1.0), "+y": ("wall_eq", None, 1.0)})
     lat.close()
+
+
+def _halo_cases():
+    """(shape, faces, solid) for the LDS x-halo checks: x extents that end
+    inside a 256-cell row tile, periodic and walled x, solids next to the
+    tile edges, NEEM faces, D2Q9 and D3Q19."""
+    walls3 = {k: ("wall", None, 1.0) for k in ["-y", "+y", "-z", "+z"]}
+    neem3 = dict(walls3)
+    neem3["-x"] = ("velocity_neem", (0.04, 0.0, 0.0), 1.0)
+    neem3["+x"] = ("pressure_neem", None, 1.0)
+    xwalls3 = dict(walls3)
+    xwalls3["-x"] = ("wall", (0.0, 0.02, 0.0), 1.0)
+    xwalls3["+x"] = ("wall", None, 1.0)
+    return {
+        "box3d_periodic": ((300, 12, 10), None, None),
+        "sphere3d_neem": ((290, 16, 14), neem3, sphere_solid((290, 16, 14), (255.5, 8, 7), 5.0)),
+        "box3d_xwalls": ((257, 10, 8), xwalls3, sphere_solid((257, 10, 8), (0.5, 5, 4), 3.0)),
+        "channel2d_neem": ((520, 24), {"-y": ("wall", None, 1.0), "+y": ("wall", (0.01, 0.0), 1.0),
+                                       "-x": ("velocity_neem", (0.03, 0.0), 1.0), "+x": ("pressure_neem", None, 1.0)},
+                           cylinder_solid2((520, 24), (256.0, 12.0), 5.0)),
+    }
+
+
+@pytest.mark.parametrize("case", ["box3d_periodic", "sphere3d_neem", "box3d_xwalls", "channel2d_neem"])
+def test_lds_halo_gather_bitwise_equal(case):
+    """lbm_set_halo(1) (populations with c_x != 0 staged through LDS per
+    256-cell row) against the default gather from global memory: identical
+    populations after plain, snapshot and check steps, identical sums."""
+    shape, faces, solid = _halo_cases()[case]
+    dim = len(shape)
+    N = int(np.prod(shape))
+    rng = np.random.default_rng(4)
+    rho0 = 1 + 0.01 * rng.standard_normal(N)
+    u0 = 0.02 * rng.standard_normal((N, dim))
+    if solid is not None:
+        u0[solid] = 0.0
+    print(f"lds halo case {case} shape {shape}", flush=True)
+    out = []
+    for lds in (0, 1):
+        lat = capi.Lattice(shape, storage="f32", omega=1 / 0.56, faces=faces, solid=solid)
+        lat.set_halo(lds)
+        lat.init(rho0, u0)
+        lat.step(7)
+        s1, rc1 = lat.step_check()
+        lat.step(3)
+        s2, rc2 = lat.step_ex(3)
+        assert rc1 == 0 and rc2 == 0
+        out.append((lat.populations(), s1.as_dict(), s2.as_dict()))
+        lat.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+
+
+def test_lds_halo_matches_oracle_sphere_channel():
+    """The LDS x-halo kernel against the FP64 oracle on a sphere channel with
+    NEEM inlet and outlet (FP32 bar)."""
+    shape, faces, solid = _halo_cases()["sphere3d_neem"]
+    N = int(np.prod(shape))
+    rho0 = np.ones(N)
+    u0 = np.zeros((N, 3))
+    u0[~solid, 0] = 0.04
+    lat = capi.Lattice(shape, storage="f32", omega=1 / 0.56, faces=faces, solid=solid)
+    lat.set_halo(1)
+    lat.init(rho0, u0)
+    lat.step(60)
+    rho, u = lat.macros()
+    lat.close()
+    ref = O.Problem(shape, faces=faces, solid=solid, omega=1 / 0.56)
+    ref.init(rho0, u0)
+    ref.step(60)
+    r_rho, r_u = ref.macros()
+    assert rel(u, r_u) < 1e-5 and rel(rho, r_rho) < 1e-5
+
+
+def test_lds_halo_rejected_where_unsupported():
+    lat = capi.Lattice((100, 12, 10), storage="f32", omega=1.0)  # 128-wide blocks
+    with pytest.raises(capi.LBMError, match="LDS x halo"):
+        lat.set_halo(1)
+    lat.close()
+    lat = capi.Lattice((300, 12, 10), storage="f64", omega=1.0)
+    with pytest.raises(capi.LBMError, match="LDS x halo"):
+        lat.set_halo(1)
+    lat.close()
