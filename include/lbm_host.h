@@ -54,6 +54,18 @@ void lbm_host_grid_destroy(lbm_host_grid *g);
 int lbm_host_run_file(const char *config_file, const char *storage, int64_t n, double *rho, double *u,
                       int64_t *steps_done);
 
+/* Value of a boundary-value expression ("cos(pi*x)", numbers, x y z, pi,
+ * + - * / ^, parentheses, sin cos tan sinh cosh tanh exp log sqrt abs asin
+ * acos atan pow atan2 min max) at (x, y, z): the subset of exprtk the
+ * reference's configurations use (src/common/mathexpr.h:8-40).  Returns 0,
+ * or -1 with lbm_host_last_error() on a syntax error. */
+int lbm_host_eval_expression(const char *text, double x, double y, double z, double *out);
+
+/* The same for a configuration whose solver "equation" is "poisson" (the
+ * reference's Poisson LBMSolver, src/lbm/solverExe.h:37-80): the electric
+ * potential after the run in reference cell order into phi[n]. */
+int lbm_host_run_poisson_file(const char *config_file, int64_t n, double *phi, int64_t *steps_done);
+
 /* Write <file_base>.vtp, the reference's base64 binary point file
  * (VTK::BINARY::writePoints, src/common/IO.h:414-507, as LBMSolver::output
  * calls it, src/lbm/solver.cpp:335-381): n points with dim coordinates

@@ -26,10 +26,10 @@ def run(cmd, cwd=None):
 
 
 # translation units of liblbm_hip.so: the API and set-up kernels, the step
-# kernels of each storage type and the warm-start network (compiled in
-# parallel)
+# kernels of each storage type, the warm-start network and the Poisson
+# solver (compiled in parallel)
 HIP_SOURCES = ["lbm_hip.hip", "lbm_step_f32.hip", "lbm_step_f64.hip", "lbm_step_f16.hip", "lbm_step_bf16.hip",
-               "lbm_surrogate.hip"]
+               "lbm_surrogate.hip", "lbm_poisson.hip"]
 
 
 def build_hip(extra_flags=(), out=None, jobs=None):

@@ -60,7 +60,7 @@ auto dirVec(GInt dim, GInt dir) -> std::array<GDouble, 3> {
 
 void CartesianGrid::generate(Configuration& cfg) {
   m_dim = (int)cfg.required_config_value<GInt>("dim");
-  if (m_dim < 2 || m_dim > 3) TERMM(-1, "Only 2D and 3D grids are supported by this build");
+  if (m_dim < 1 || m_dim > 3) TERMM(-1, "Only 1D, 2D and 3D grids are supported");
   const GInt maxNoCells = cfg.required_config_value<GInt>("maxNoCells");
   const GInt partitionLvl = cfg.required_config_value<GInt>("partitionLevel");
   const GInt uniformLvl = cfg.required_config_value<GInt>("uniformLevel");
@@ -116,7 +116,7 @@ void CartesianGrid::generate(Configuration& cfg) {
   }
   auto centreOf = [&](GInt l, GInt i, GInt j, GInt k, GDouble* c) {
     c[0] = cen[l][i][0];
-    c[1] = cen[l][j][1];
+    c[1] = m_dim >= 2 ? cen[l][j][1] : 0.0;
     c[2] = m_dim == 3 ? cen[l][k][2] : 0.0;
   };
 
@@ -139,7 +139,7 @@ void CartesianGrid::generate(Configuration& cfg) {
     for (GInt k = wc.lo[2]; k < wc.lo[2] + wc.n[2]; ++k)
       for (GInt j = wc.lo[1]; j < wc.lo[1] + wc.n[1]; ++j)
         for (GInt i = wc.lo[0]; i < wc.lo[0] + wc.n[0]; ++i) {
-          const GInt pi = i >> 1, pj = j >> 1, pk = m_dim == 3 ? (k >> 1) : 0;
+          const GInt pi = i >> 1, pj = m_dim >= 2 ? (j >> 1) : 0, pk = m_dim == 3 ? (k >> 1) : 0;
           const GInt p = w.idx(pi, pj, pk);
           if (!alive[p]) continue;
           const GInt c = wc.idx(i, j, k);
@@ -438,7 +438,7 @@ void CartesianGrid::addDiagonalNghbrs() {
       m_nghbr[q * ns + 5] = tmpN(pX, 2);
       m_nghbr[q * ns + 6] = tmpN(mX, 2);
       m_nghbr[q * ns + 7] = tmpN(mX, 3);
-    } else {
+    } else if (m_dim == 3) {
       for (GInt dir = 6; dir < 26; ++dir) {
         const auto v = dirVec(3, dir);
         GInt c = q;

@@ -2,6 +2,8 @@
 #include <cstring>
 
 #include "../../include/lbm_host.h"
+#include "expr.h"
+#include "poisson.h"
 #include "solver.h"
 #include "vtk.h"
 
@@ -121,6 +123,36 @@ int lbm_host_run_file(const char* config_file, const char* storage, int64_t n, d
       for (GInt a = 0; a < d; ++a) u[q * d + a] = s.velocity()[q * d + a];
     }
   }
+  return (int)rc;
+}
+
+int lbm_host_eval_expression(const char* text, double x, double y, double z, double* out) {
+  if (!text || !out) {
+    g_host_err = "lbm_host_eval_expression: null argument";
+    return -1;
+  }
+  try {
+    *out = expr::Expression(text).eval(x, y, z);
+  } catch (const std::exception& e) {
+    g_host_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int lbm_host_run_poisson_file(const char* config_file, int64_t n, double* phi, int64_t* steps_done) {
+  SolverOptions opt;
+  opt.quiet = true;
+  GridGenerator gen;
+  gen.init(0, nullptr, config_file);
+  if (gen.run() != 0) return -1;
+  PoissonSolver s(opt);
+  s.init(0, nullptr, config_file);
+  s.takeGrid(gen.releaseGrid());
+  const GInt rc = s.run();
+  if (steps_done) *steps_done = s.stepsDone();
+  if (phi && n == (int64_t)s.potential().size())
+    for (GInt q = 0; q < n; ++q) phi[q] = s.potential()[q];
   return (int)rc;
 }
 

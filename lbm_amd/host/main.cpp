@@ -8,6 +8,7 @@
 // Exit status: sum of the runners' returns; fatal errors exit 255.
 #include <cstring>
 
+#include "poisson.h"
 #include "solver.h"
 
 using namespace lbmh;
@@ -79,10 +80,19 @@ int main(int argc, char** argv) {
   gen.init(argc, argv, config);
   GInt ret = gen.run();
   if (ret == 0 && runLBM) {
-    LBMSolver s(opt);
-    s.init(argc, argv, config);
-    s.takeGrid(gen.releaseGrid());
-    ret += s.run();
+    // LBMSolverExecutor::init (src/lbm/solverExe.h:17-93): the solver
+    // section's "equation" picks the Navier-Stokes or the Poisson solver
+    if (solverEquation(cfg) == "poisson") {
+      PoissonSolver s(opt);
+      s.init(argc, argv, config);
+      s.takeGrid(gen.releaseGrid());
+      ret += s.run();
+    } else {
+      LBMSolver s(opt);
+      s.init(argc, argv, config);
+      s.takeGrid(gen.releaseGrid());
+      ret += s.run();
+    }
   }
   return (int)ret;
 }

@@ -477,7 +477,7 @@ class SolverGrid:
             if self.dim == 2:
                 mX, pX = main[c][0], main[c][1]
                 row += [t(pX, 3), t(pX, 2), t(mX, 2), t(mX, 3)]
-            else:
+            elif self.dim == 3:
                 for d in range(6, 26):
                     v = DIR_VEC3[d]
                     cur = c

@@ -4,7 +4,7 @@ where the reference is mounted; the fixtures themselves are committed).
 hilbert_kat.json   <- UnitTest/test_hilbert.cpp (known answers of hilbert::index)
 lattice_tables.json <- src/lbm/constants.h (LBMethod<D2Q9>/<D3Q19> velocity,
                        weight and opposite tables, parsed as data)
-reference_configs/  <- test/{couette,poiseuille}/*.json, test/sphere, test/step
+reference_configs/  <- test/{couette,poiseuille,poisson}/*.json, test/sphere, test/step
                        (the reference's own integration-test inputs)
 """
 import json
@@ -53,7 +53,7 @@ def tables():
 def configs():
     dst = os.path.join(OUT, "reference_configs")
     os.makedirs(dst, exist_ok=True)
-    for sub in ("couette", "poiseuille", "sphere", "step"):
+    for sub in ("couette", "poiseuille", "poisson", "sphere", "step"):
         for f in sorted(os.listdir(os.path.join(REF, "test", sub))):
             if f.endswith(".json"):
                 shutil.copy(os.path.join(REF, "test", sub, f), os.path.join(dst, f"{sub}__{f}"))

@@ -17,6 +17,7 @@ def declared(header):
 
 @pytest.mark.parametrize("header,lib", [("lbm_hip.h", "lbm_amd/liblbm_hip.so"), ("lbm_batch.h", "lbm_amd/liblbm_hip.so"),
                                         ("lbm_surrogate.h", "lbm_amd/liblbm_hip.so"),
+                                        ("lbm_poisson.h", "lbm_amd/liblbm_hip.so"),
                                         ("lbm_host.h", "lbm_amd/liblbm_host.so")])
 def test_library_exports_header(header, lib):
     path = os.path.join(ROOT, lib)
@@ -36,12 +37,13 @@ def test_python_binding_lists_all_exports():
 
 
 def test_python_bindings_list_batch_and_surrogate_exports():
-    from lbm_amd import capi, surrogate
+    from lbm_amd import capi, poisson, surrogate
 
     assert sorted(capi.BATCH_EXPORTS) == declared("lbm_batch.h")
     assert sorted(surrogate.EXPORTS) == declared("lbm_surrogate.h")
+    assert sorted(poisson.EXPORTS) == declared("lbm_poisson.h")
 
 
 def test_every_header_is_checked():
     assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
-        ["lbm_batch.h", "lbm_hip.h", "lbm_host.h", "lbm_surrogate.h"]
+        ["lbm_batch.h", "lbm_hip.h", "lbm_host.h", "lbm_poisson.h", "lbm_surrogate.h"]
