@@ -68,19 +68,26 @@ struct ConvArgs {
   int cf;
 };
 
+// One 32-pixel x 32-output-channel tile per `ksplit` waves of a block (1, 2
+// or 4): the K steps (tap, 16 channels) are split into ksplit contiguous
+// ranges, each wave issues the next step's two loads before this step's
+// product, and the partial tiles are added in wave order through LDS by the
+// tile's first wave, which applies the epilogue.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_conv(const ConvArgs a) {
+__global__ void __launch_bounds__(256) k_conv(const ConvArgs a, const int ksplit) {
+  __shared__ float red[4][16][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long npix = (long long)a.nb * a.h * a.wd;
   const int ntc = a.co >> 5;
-  const long long tile = (long long)blockIdx.x * 4 + wave;
+  const int kp = wave % ksplit;  // this wave's part of K
+  const long long tile = (long long)blockIdx.x * (4 / ksplit) + wave / ksplit;
   const long long p0 = (tile / ntc) * 32;
-  if (p0 >= npix) return;  // wave-uniform; no block barrier in this kernel
+  const bool active = p0 < npix;  // uniform over the waves of a tile
   const int co0 = (int)(tile % ntc) * 32;
   const int ci = a.ca + a.cs;
   const int m = lane & 31, g = lane >> 5;
   const long long p = p0 + m;
-  const bool pin = p < npix;
+  const bool pin = active && p < npix;
   int b = 0, y = 0, x = 0;
   if (pin) {
     x = (int)(p % a.wd);
@@ -92,33 +99,133 @@ __global__ void __launch_bounds__(256) k_conv(const ConvArgs a) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
   const int h2 = a.h >> 1, w2 = a.wd >> 1;
-  for (int tap = 0; tap < 9; ++tap) {
+  const int cpt = ci >> 4;  // K steps per tap
+  const int nks = 9 * cpt;
+  const int k0 = kp * nks / ksplit, k1 = (kp + 1) * nks / ksplit;
+  const u16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  // operands of K step k: 8 input channels of this lane's pixel (shifted by
+  // the tap) and 8 weights of its output channel
+  auto load = [&](int k, u16x8 &av, u16x8 &bv) {
+    const int tap = k / cpt, c = (k - tap * cpt) << 4;
     const int ys = y + tap / 3 - 1, xs = x + tap % 3 - 1;
-    const bool ok = pin && ys >= 0 && ys < a.h && xs >= 0 && xs < a.wd;
-    const u16 *wt = wrow + tap * ci;
-    if (a.ca > 0) {
-      const u16 *src = a.a + (((long long)b * h2 + (ys >> 1)) * w2 + (xs >> 1)) * a.ca + 8 * g;
-      for (int c = 0; c < a.ca; c += 16) {
-        bf16x8 av;
-        if (ok) av = ld8(src + c);
-        else av = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(wt + c), acc, 0, 0, 0);
-      }
+    bv = *reinterpret_cast<const u16x8 *>(wrow + tap * ci + c);
+    av = zero;
+    if (pin && ys >= 0 && ys < a.h && xs >= 0 && xs < a.wd) {
+      if (c < a.ca)
+        av = *reinterpret_cast<const u16x8 *>(a.a + (((long long)b * h2 + (ys >> 1)) * w2 + (xs >> 1)) * a.ca + c + 8 * g);
+      else
+        av = *reinterpret_cast<const u16x8 *>(a.s + (((long long)b * a.h + ys) * a.wd + xs) * a.cs + (c - a.ca) + 8 * g);
     }
-    const u16 *src = a.s + (((long long)b * a.h + ys) * a.wd + xs) * a.cs + 8 * g;
-    for (int c = 0; c < a.cs; c += 16) {
-      bf16x8 av;
-      if (ok) av = ld8(src + c);
-      else av = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, ld8(wt + a.ca + c), acc, 0, 0, 0);
+  };
+  if (active && k0 < k1) {
+    u16x8 av, bv;
+    load(k0, av, bv);
+    for (int k = k0; k < k1; ++k) {
+      u16x8 av1 = zero, bv1 = zero;
+      if (k + 1 < k1) load(k + 1, av1, bv1);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc,
+                                                    0, 0, 0);
+      av = av1;
+      bv = bv1;
     }
   }
+  if (ksplit > 1) {
+    if (kp > 0)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[wave][r][lane] = acc[r];
+    __syncthreads();
+    if (kp > 0) return;
+    for (int q = 1; q < ksplit; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] += red[wave + q][r][lane];
+  }
+  if (!active) return;
   // lane holds C[mm][n], n = lane % 32, mm = 8 (r / 4) + 4 (lane / 32) + r % 4
   const int co_ = co0 + m;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const long long pp = p0 + ((r >> 2) << 3) + 4 * g + (r & 3);
     if (pp >= npix) continue;
+    float v = acc[r];
+    if (MODE == 0) {
+      v += a.bias[co_];
+      if (a.relu) v = fmaxf(v, 0.f);
+      a.y[pp * a.co + co_] = f2bf(v);
+    } else if (MODE == 1) {
+      if (co_ < a.c0) a.g0[pp * a.c0 + co_] += v;
+      else a.g1[pp * (a.co - a.c0) + (co_ - a.c0)] += v;
+    } else {
+      if (co_ < a.cf) a.yf[pp * a.cf + co_] = v + a.bias[co_];
+    }
+  }
+}
+
+// The same convolution with the input staged in LDS (levels whose width is
+// a multiple of 32 and ci <= LDS_CI_MAX): a block computes 4 image rows x
+// 32 pixels x 32 output channels (blockIdx.y = output tile), one wave per
+// row.  The block first loads the 6 x 34 input pixels of its rows and their
+// one-pixel halo (zero outside the image; the upsampled channels first, read
+// at half resolution) with 16-byte loads, then every tap reads its A operand
+// from LDS, so each input element comes from HBM/L2 once per block instead
+// of once per tap.  Pixel stride in LDS: ci + 8 elements.
+constexpr int LDS_CI_MAX = 128;
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv_lds(const ConvArgs a) {
+  extern __shared__ __align__(16) u16 xs[];  // [6][34][ci + 8]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = a.ca + a.cs, cst = ci + 8;
+  const int xt = a.wd >> 5, ry = (a.h + 3) >> 2;
+  const int blk = blockIdx.x;
+  const int x0 = (blk % xt) * 32;
+  const int y0 = ((blk / xt) % ry) * 4;
+  const int b = blk / (xt * ry);
+  const int co0 = blockIdx.y * 32;
+  const int h2 = a.h >> 1, w2 = a.wd >> 1;
+  // stage rows y0-1 .. y0+4, columns x0-1 .. x0+32, 8 channels per load
+  const int cg = ci >> 3;
+  for (int e = threadIdx.x; e < 6 * 34 * cg; e += 256) {
+    const int c = (e % cg) << 3;
+    const int pix = e / cg;
+    const int yy = y0 - 1 + pix / 34, xx = x0 - 1 + pix % 34;
+    u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.wd) {
+      if (c < a.ca)
+        v = *reinterpret_cast<const u16x8 *>(a.a + (((long long)b * h2 + (yy >> 1)) * w2 + (xx >> 1)) * a.ca + c);
+      else
+        v = *reinterpret_cast<const u16x8 *>(a.s + (((long long)b * a.h + yy) * a.wd + xx) * a.cs + (c - a.ca));
+    }
+    *reinterpret_cast<u16x8 *>(xs + pix * cst + c) = v;
+  }
+  __syncthreads();
+  const int y = y0 + wave;
+  if (y >= a.h) return;  // after the only barrier
+  const int m = lane & 31, g = lane >> 5;
+  const u16 *wrow = a.w + (long long)(co0 + m) * 9 * ci + 8 * g;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  const int cpt = ci >> 4, nks = 9 * cpt;
+  u16x8 bv = *reinterpret_cast<const u16x8 *>(wrow);
+  for (int k = 0; k < nks; ++k) {
+    const int tap = k / cpt, c = (k - tap * cpt) << 4;
+    u16x8 bv1 = bv;
+    if (k + 1 < nks) {
+      const int t1 = (k + 1) / cpt, c1 = ((k + 1) - t1 * cpt) << 4;
+      bv1 = *reinterpret_cast<const u16x8 *>(wrow + t1 * ci + c1);
+    }
+    // pixel (y + dy, x0 + m + dx) sits at LDS row wave + 1 + dy, column m + 1 + dx
+    const int pix = (wave + tap / 3) * 34 + m + tap % 3;
+    const u16x8 av = *reinterpret_cast<const u16x8 *>(xs + pix * cst + c + 8 * g);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc,
+                                                  0, 0, 0);
+    bv = bv1;
+  }
+  // lane holds C[mm][n], n = lane % 32, mm = 8 (r / 4) + 4 (lane / 32) + r % 4
+  const int co_ = co0 + m;
+  const long long prow = ((long long)b * a.h + y) * a.wd + x0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long long pp = prow + ((r >> 2) << 3) + 4 * g + (r & 3);
     float v = acc[r];
     if (MODE == 0) {
       v += a.bias[co_];
@@ -242,22 +349,265 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
   }
 }
 
-// dst[i] = sum over chunks of part[chunk][i], chunks in order
-__global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long long n, int nchunks, float *dst) {
-  const long long i = blockIdx.x * 256LL + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.f;
-  int c = 0;
-  // eight loads in flight, added in chunk order
-  for (; c + 8 <= nchunks; c += 8) {
-    float v[8];
+// ---------------------------------------------------------------------------
+// Weight gradient on channel-major operands.  The pixel sum is the GEMM's K
+// axis, and NHWC keeps pixels strided; two staging kernels write the layer's
+// output gradient as gT[co][p] and its input, zero padded, as
+// xT[ci][b][h + 2][w + 8] (one pixel of halo on each side, the rows padded
+// to a multiple of 8 elements), both through an LDS tile transpose, so the
+// MFMA operands of k_wgrad_t are 16-byte loads of 8 consecutive pixels.
+// Used where the width at the layer's level is a multiple of 8.
+constexpr int TP = 64;  // pixels per transpose tile
+
+// From the FP32 gradient g [p][co] of a layer's output (relu mask from its
+// bf16 activation y), or from a bf16 gradient gin: gb = bf16 values [p][co]
+// (not written when gin is given), gT [co][p], and per-tile bias sums
+// bpart[tile][co] of the bf16 values (8-pixel groups, then the groups in
+// order).  Block: 64 pixels x 32 channels; grid (ceil(P / 64), co / 32).
+__global__ void __launch_bounds__(256) k_gy_prep(const float *g, const u16 *y, int relu, const u16 *gin, u16 *gb,
+                                                 u16 *gT, float *bpart, long long P, int co) {
+  __shared__ u16 tile[32][TP + 8];
+  __shared__ float red[8][32];
+  const int t = threadIdx.x;
+  const int px = t >> 2, k = t & 3;
+  const long long p0 = (long long)blockIdx.x * TP;
+  const int c0 = blockIdx.y * 32;
+  const long long p = p0 + px;
+  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (p < P) {
+    const long long off = p * co + c0 + 8 * k;
+    if (gin) {
+      v = *reinterpret_cast<const u16x8 *>(gin + off);
+    } else {
+      const float4 lo = *reinterpret_cast<const float4 *>(g + off);
+      const float4 hi = *reinterpret_cast<const float4 *>(g + off + 4);
+      float f[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      u16x8 yy = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (relu) yy = *reinterpret_cast<const u16x8 *>(y + off);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = part[(long long)(c + k) * n + i];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s += v[k];
+      for (int i = 0; i < 8; ++i) {
+        if (relu && !(bf2f(yy[i]) > 0.f)) f[i] = 0.f;
+        v[i] = f2bf(f[i]);
+      }
+      *reinterpret_cast<u16x8 *>(gb + off) = v;
+    }
   }
-  for (; c < nchunks; ++c) s += part[(long long)c * n + i];
-  dst[i] = s;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) tile[8 * k + i][px] = v[i];
+  __syncthreads();
+  const int ch = t >> 3, j = t & 7;
+  const long long pp = p0 + 8 * j;
+  u16x8 w;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = tile[ch][8 * j + i];
+  if (pp < P) *reinterpret_cast<u16x8 *>(gT + (long long)(c0 + ch) * P + pp) = w;
+  float sum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) sum += bf2f(w[i]);  // zero beyond P
+  red[j][ch] = sum;
+  __syncthreads();
+  if (t < 32) {
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += red[q][t];
+    bpart[(long long)blockIdx.x * co + c0 + t] = acc;
+  }
+}
+
+// xT[c][b][yp][xp] = input channel c of pixel (b, yp - 1, xp - 1), zero
+// outside the image; channels < ca come from the half-resolution tensor A
+// (nearest upsampling), the rest from S.  Block: 64 padded columns x 32
+// channels of one padded row; grid (ceil((w + 8) / 64), nb (h + 2),
+// ceil(ci / 32)).  Requires ca, cs multiples of 8.
+__global__ void __launch_bounds__(256) k_x_prep(const u16 *A, int ca, const u16 *S, int cs, int nb, int h, int w,
+                                                u16 *xT) {
+  __shared__ u16 tile[32][TP + 8];
+  const int ci = ca + cs, wp = w + 8, hp = h + 2;
+  const int t = threadIdx.x, xl = t >> 2, k = t & 3;
+  const int row = blockIdx.y;  // b * hp + yp
+  const int b = row / hp, yp = row % hp;
+  const int c0 = blockIdx.z * 32, c = c0 + 8 * k;
+  const int x = (int)blockIdx.x * TP + xl - 1, yy = yp - 1;
+  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (c < ci && yy >= 0 && yy < h && x >= 0 && x < w) {
+    if (c < ca)
+      v = *reinterpret_cast<const u16x8 *>(A + (((long long)b * (h >> 1) + (yy >> 1)) * (w >> 1) + (x >> 1)) * ca + c);
+    else
+      v = *reinterpret_cast<const u16x8 *>(S + (((long long)b * h + yy) * w + x) * cs + (c - ca));
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) tile[8 * k + i][xl] = v[i];
+  __syncthreads();
+  const int ch = t >> 3, j = t & 7;
+  const int xq = (int)blockIdx.x * TP + 8 * j;
+  if (c0 + ch < ci && xq < wp) {
+    u16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = tile[ch][8 * j + i];
+    *reinterpret_cast<u16x8 *>(xT + ((long long)(c0 + ch) * nb * hp + row) * wp + xq) = o;
+  }
+}
+
+struct WgradTArgs {
+  const u16 *gT;  // [co][P]
+  const u16 *xT;  // [ci][nb][h + 2][w + 8]
+  long long P;
+  int co, ci, nb, h, w;
+  int cseg;     // 8-pixel segments per chunk (even)
+  int nchunks;
+  float *part;  // [chunk][co][9][ci]
+};
+
+// 8 bf16 values starting `sh` elements (0, 1 or 2) into lo:hi
+__device__ __forceinline__ bf16x8 shifted8(const u16x8 lo, const u16x8 hi, int sh) {
+  typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 a = __builtin_bit_cast(u32x4, lo), b = __builtin_bit_cast(u32x4, hi);
+  const unsigned d[5] = {a[0], a[1], a[2], a[3], b[0]};
+  u32x4 r;
+  if (sh == 0) {
+    r = a;
+  } else if (sh == 2) {
+    r = u32x4{d[1], d[2], d[3], d[4]};
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) r[q] = (unsigned)((((unsigned long long)d[q + 1] << 32) | d[q]) >> 16);
+  }
+  return __builtin_bit_cast(bf16x8, r);
+}
+
+// One wave: (co tile, ci tile, tap row dy, chunk of cseg 8-pixel segments),
+// the three taps of the row (three 32x32 accumulators).  A K step is two
+// segments (lane group g takes segment 2k + g; a segment never crosses an
+// image row since w % 8 == 0): the gradient operand is one 16-byte load,
+// the input row two aligned loads serving the three dx shifts; the next
+// step's loads are issued before this step's products.  The chunk sums in
+// pixel order, so the gradient is the same bit for bit however the loads
+// are scheduled.
+__global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntco = a.co >> 5, ntci = (a.ci + 31) >> 5;
+  long long t = (long long)blockIdx.x * 4 + wave;
+  if (t >= (long long)ntco * ntci * 3 * a.nchunks) return;
+  const int chunk = (int)(t % a.nchunks);
+  t /= a.nchunks;
+  const int dy = (int)(t % 3);
+  t /= 3;
+  const int cit = (int)(t % ntci), cot = (int)(t / ntci);
+  const int co0 = cot * 32, ci0 = cit * 32;
+  const int m = lane & 31, g = lane >> 5;
+  const int wp = a.w + 8, hp = a.h + 2;
+  const bool cok = ci0 + m < a.ci;
+  const u16 *arow = a.gT + (long long)(co0 + m) * a.P;
+  const u16 *xch = a.xT + (long long)(cok ? ci0 + m : 0) * a.nb * hp * wp + (long long)dy * wp;
+  const long long nseg = a.P >> 3;
+  const long long sbeg = (long long)chunk * a.cseg;
+  const long long send = sbeg + a.cseg < nseg ? sbeg + a.cseg : nseg;
+  f32x16 acc[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[q][r] = 0.f;
+  // pixel of this lane's segment, advanced by carries
+  long long s = sbeg + g;
+  int x, y, b;
+  {
+    const long long p = s << 3;
+    x = (int)(p % a.w);
+    const long long r = p / a.w;
+    y = (int)(r % a.h);
+    b = (int)(r / a.h);
+  }
+  const u16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto load = [&](u16x8 &av, u16x8 &lo, u16x8 &hi) {
+    av = zero;
+    lo = zero;
+    hi = zero;
+    if (s < send) {
+      av = *reinterpret_cast<const u16x8 *>(arow + (s << 3));
+      if (cok) {
+        const u16 *xr = xch + ((long long)b * hp + y) * wp + x;
+        lo = *reinterpret_cast<const u16x8 *>(xr);
+        hi = *reinterpret_cast<const u16x8 *>(xr + 8);
+      }
+    }
+    s += 2;
+    x += 16;
+    while (x >= a.w) {
+      x -= a.w;
+      if (++y == a.h) {
+        y = 0;
+        ++b;
+      }
+    }
+  };
+  u16x8 av, lo, hi;
+  load(av, lo, hi);
+  for (long long sk = sbeg; sk < send; sk += 2) {  // wave-uniform
+    u16x8 av1 = zero, lo1 = zero, hi1 = zero;
+    if (sk + 2 < send) load(av1, lo1, hi1);
+    const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx)
+      acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, shifted8(lo, hi, dx), acc[dx], 0, 0, 0);
+    av = av1;
+    lo = lo1;
+    hi = hi1;
+  }
+  if (!cok) return;
+  float *dst = a.part + (long long)chunk * a.co * 9 * a.ci;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int mm = ((r >> 2) << 3) + 4 * g + (r & 3);
+      dst[((long long)(co0 + mm) * 9 + 3 * dy + q) * a.ci + ci0 + m] = acc[q][r];
+    }
+}
+
+// dst[c] = sum over rows r of part[r][c] (n columns): block c, thread t sums
+// rows t, t + 256, ... in order, then a fixed tree
+__global__ void __launch_bounds__(256) k_reduce_rows(const float *part, long long nrows, int n, float *dst) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long long r = threadIdx.x; r < nrows; r += 256) s += part[r * n + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dst[c] = red[0];
+}
+
+// dst[i] = sum over chunks of part[chunk][i]: a block takes 32 elements,
+// its eight 32-lane groups each sum one eighth of the chunks in order, and
+// the eight partial sums are added in group order
+__global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long long n, int nchunks, float *dst) {
+  __shared__ float red[8][32];
+  const int e = threadIdx.x & 31, q = threadIdx.x >> 5;
+  const long long i = blockIdx.x * 32LL + e;
+  float s = 0.f;
+  if (i < n) {
+    const int c1 = (int)((long long)(q + 1) * nchunks / 8);
+    int c = (int)((long long)q * nchunks / 8);
+    for (; c + 4 <= c1; c += 4) {
+      float v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = part[(long long)(c + k) * n + i];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += v[k];
+    }
+    for (; c < c1; ++c) s += part[(long long)c * n + i];
+  }
+  red[q][e] = s;
+  __syncthreads();
+  if (q == 0 && i < n) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += red[k][e];
+    dst[i] = t;
+  }
 }
 
 // bias gradient partials: part[chunk][c] = sum over the chunk's pixels of
@@ -398,18 +748,58 @@ __global__ void __launch_bounds__(256) k_adam(float *p, const float *g, float *m
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// bf16 working copies of one layer: wb[co][tap][ci] and the dgrad form
-// wd[ci][8 - tap][co]
-__global__ void __launch_bounds__(256) k_prep_w(const float *wf, int co, int ci, u16 *wb, u16 *wd) {
+// bf16 working copies of every layer in one launch (blockIdx.y = layer):
+// wb[co][tap][ci] and the dgrad form wd[ci][8 - tap][co]
+constexpr int MAXLAYERS = 32;
+struct PrepLayer {
+  const float *wf;
+  int co, ci;
+  u16 *wb, *wd;
+};
+struct PrepAll {
+  PrepLayer l[MAXLAYERS];
+  int n;
+};
+__global__ void __launch_bounds__(256) k_prep_all(const PrepAll a) {
+  const PrepLayer &L = a.l[blockIdx.y];
   const long long i = blockIdx.x * 256LL + threadIdx.x;
-  const long long n = (long long)co * 9 * ci;
+  const long long n = (long long)L.co * 9 * L.ci;
   if (i >= n) return;
-  const int c = (int)(i % ci);
-  const int tap = (int)((i / ci) % 9);
-  const int o = (int)(i / (9LL * ci));
-  const u16 v = f2bf(wf[i]);
-  wb[i] = v;
-  if (wd) wd[((long long)c * 9 + (8 - tap)) * co + o] = v;
+  const int c = (int)(i % L.ci);
+  const int tap = (int)((i / L.ci) % 9);
+  const int o = (int)(i / (9LL * L.ci));
+  const u16 v = f2bf(L.wf[i]);
+  L.wb[i] = v;
+  L.wd[((long long)c * 9 + (8 - tap)) * L.co + o] = v;
+}
+
+// one batch from the device dataset: features, targets and masks of
+// samples idx[0..nb) (blockIdx.y = k) into the batch buffers
+constexpr int MAXGATHER = 64;
+struct GatherArgs {
+  long long idx[MAXGATHER];
+  long long ps;  // pixels per sample
+  int cin;
+  const float *dx, *dt;
+  const unsigned char *dm;
+  float *xin, *tgt;
+  unsigned char *mask;
+};
+__global__ void __launch_bounds__(256) k_gather(const GatherArgs a) {
+  const int k = blockIdx.y;
+  const long long i = a.idx[k];
+  const long long nx = a.ps * a.cin, nt = a.ps * 3, n = nx + nt + a.ps;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += gridDim.x * 256LL) {
+    if (e < nx) {
+      a.xin[k * nx + e] = a.dx[i * nx + e];
+    } else if (e < nx + nt) {
+      const long long f = e - nx;
+      a.tgt[k * nt + f] = a.dt[i * nt + f];
+    } else {
+      const long long f = e - nx - nt;
+      a.mask[k * a.ps + f] = a.dm[i * a.ps + f];
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256) k_fill(float *p, long long n, float v) {
@@ -418,6 +808,7 @@ __global__ void __launch_bounds__(256) k_fill(float *p, long long n, float v) {
 }
 
 inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+inline unsigned nblk32(long long n) { return (unsigned)((n + 31) / 32); }
 
 }  // namespace sur
 
@@ -463,6 +854,11 @@ struct lbm_sur {
   long long part_cap = 0;
   float *gup = nullptr;   // full-resolution gradient of an upsampled input
   long long gup_cap = 0;
+  float *grad_arena = nullptr;  // every Tensor::grad, B samples each
+  long long grad_n = 0;
+  u16 *gT = nullptr, *xT = nullptr;  // channel-major operands of k_wgrad_t
+  float *bpart = nullptr;            // bias partial sums of k_gy_prep
+  long long gT_cap = 0, xT_cap = 0, bpart_cap = 0;
   double *lpart = nullptr, *lsum = nullptr;
   // dataset
   float *dx = nullptr, *dt = nullptr;
@@ -546,11 +942,27 @@ static void build_net(lbm_sur *s) {
   s->nparam = off;
 }
 
+// the weight gradient runs on channel-major operands (k_wgrad_t) where the
+// level's width is a multiple of 8 (a segment of 8 pixels stays in one row)
+// and the channel counts are multiples of 8; else on k_wgrad
+static bool wgrad_transposed(const Layer &l, int w) { return w % 8 == 0 && l.ca % 8 == 0 && l.cs % 8 == 0; }
+
+// 8-pixel segments per k_wgrad_t chunk: up to 128 (1024 pixels), fewer on
+// the small levels so that a launch has about 2048 waves, at least 8 (a
+// wave's serial chain of loads and products stays short)
+static int wgrad_cseg(long long np, int co, int ci) {
+  const long long nseg = np / 8;
+  const long long tiles = (long long)(co / 32) * ((ci + 31) / 32) * 3;
+  long long c = (nseg * tiles + 2047) / 2048;
+  c = std::max(8LL, std::min(128LL, c));
+  return (int)((c + 1) & ~1LL);
+}
+
 static int alloc_all(lbm_sur *s) {
   for (Tensor &T : s->t) {
     const long long n = npix_at(s, T.level, s->B) * T.c;
     SCHK(s, hipMalloc(&T.act, n * sizeof(u16)));
-    SCHK(s, hipMalloc(&T.grad, n * sizeof(float)));
+    s->grad_n += (n + 63) / 64 * 64;
   }
   for (Layer &l : s->L) {
     const long long n = (long long)l.co * 9 * (l.ca + l.cs);
@@ -560,6 +972,26 @@ static int alloc_all(lbm_sur *s) {
     const long long nch = (np + WGCHUNK - 1) / WGCHUNK;  // WCHUNK >= WGCHUNK: covers the bias partials
     s->part_cap = std::max(s->part_cap, nch * n);
     if (l.ca > 0) s->gup_cap = std::max(s->gup_cap, np * l.ca);
+    const int h = s->H >> s->t[l.s].level, w = s->W >> s->t[l.s].level;
+    if (wgrad_transposed(l, w)) {
+      for (int nb = 1; nb <= s->B; ++nb) {  // the chunking depends on the batch in use
+        const long long npb = npix_at(s, s->t[l.s].level, nb);
+        const long long cseg = wgrad_cseg(npb, l.co, l.ca + l.cs);
+        s->part_cap = std::max(s->part_cap, (npb / 8 + cseg - 1) / cseg * n);
+      }
+      s->gT_cap = std::max(s->gT_cap, np * l.co);
+      s->xT_cap = std::max(s->xT_cap, (long long)(l.ca + l.cs) * s->B * (h + 2) * (w + 8));
+      s->bpart_cap = std::max(s->bpart_cap, (np + TP - 1) / TP * l.co);
+    }
+  }
+  // one arena for every tensor's gradient: cleared by one launch per step
+  SCHK(s, hipMalloc(&s->grad_arena, s->grad_n * sizeof(float)));
+  {
+    long long off = 0;
+    for (Tensor &T : s->t) {
+      T.grad = s->grad_arena + off;
+      off += (npix_at(s, T.level, s->B) * T.c + 63) / 64 * 64;
+    }
   }
   const long long np0 = npix_at(s, 0, s->B);
   SCHK(s, hipMalloc(&s->p, s->nparam * sizeof(float)));
@@ -573,22 +1005,62 @@ static int alloc_all(lbm_sur *s) {
   SCHK(s, hipMalloc(&s->gout, np0 * 32 * sizeof(u16)));
   SCHK(s, hipMalloc(&s->part, s->part_cap * sizeof(float)));
   SCHK(s, hipMalloc(&s->gup, std::max(1LL, s->gup_cap) * sizeof(float)));
+  SCHK(s, hipMalloc(&s->gT, std::max(1LL, s->gT_cap) * sizeof(u16)));
+  SCHK(s, hipMalloc(&s->xT, std::max(1LL, s->xT_cap) * sizeof(u16)));
+  SCHK(s, hipMalloc(&s->bpart, std::max(1LL, s->bpart_cap) * sizeof(float)));
   SCHK(s, hipMalloc(&s->lpart, ((np0 + 255) / 256) * sizeof(double)));
   SCHK(s, hipMalloc(&s->lsum, sizeof(double)));
   return LBM_OK;
 }
 
 static int prep_weights(lbm_sur *s) {
-  for (Layer &l : s->L) {
-    const long long n = (long long)l.co * 9 * (l.ca + l.cs);
-    hipLaunchKernelGGL(k_prep_w, dim3(nblk(n)), dim3(256), 0, s->st, (const float *)(s->p + l.woff), l.co,
-                       l.ca + l.cs, l.wb, l.wd);
+  if ((int)s->L.size() > MAXLAYERS) return sfail(s, LBM_EINVAL, "lbm_sur: too many layers");
+  PrepAll pa;
+  long long nmax = 0;
+  pa.n = (int)s->L.size();
+  for (int i = 0; i < pa.n; ++i) {
+    const Layer &l = s->L[i];
+    pa.l[i] = PrepLayer{s->p + l.woff, l.co, l.ca + l.cs, l.wb, l.wd};
+    nmax = std::max(nmax, (long long)l.co * 9 * (l.ca + l.cs));
   }
+  hipLaunchKernelGGL(k_prep_all, dim3(nblk(nmax), pa.n), dim3(256), 0, s->st, pa);
   SCHK(s, hipGetLastError());
   return LBM_OK;
 }
 
-static unsigned conv_blocks(long long npix, int co) { return (unsigned)(((npix + 31) / 32 * (co / 32) + 3) / 4); }
+// waves per output tile of k_conv: the small levels split K over 2 or 4
+// waves so that a launch keeps enough waves in flight
+static int conv_ksplit(long long npix, int co) {
+  const long long tiles = (npix + 31) / 32 * (co / 32);
+  return tiles >= 8192 ? 1 : tiles >= 2048 ? 2 : 4;
+}
+static unsigned conv_blocks(long long npix, int co) {
+  const long long tiles = (npix + 31) / 32 * (co / 32);
+  const int per = 4 / conv_ksplit(npix, co);
+  return (unsigned)((tiles + per - 1) / per);
+}
+
+// the large levels (at least 65536 pixels, width a multiple of 32, at most
+// LDS_CI_MAX input channels) run k_conv_lds
+static bool conv_lds(long long npix, int w, int ci) { return npix >= 65536 && w % 32 == 0 && ci <= LDS_CI_MAX; }
+
+static void launch_conv(int mode, const ConvArgs &a, hipStream_t st) {
+  const long long np = (long long)a.nb * a.h * a.wd;
+  const int ci = a.ca + a.cs;
+  if (conv_lds(np, a.wd, ci)) {
+    const dim3 grid((unsigned)(a.nb * ((a.h + 3) / 4) * (a.wd / 32)), (unsigned)(a.co / 32));
+    const size_t lds = (size_t)6 * 34 * (ci + 8) * sizeof(u16);
+    if (mode == 0) hipLaunchKernelGGL(k_conv_lds<0>, grid, dim3(256), lds, st, a);
+    else if (mode == 1) hipLaunchKernelGGL(k_conv_lds<1>, grid, dim3(256), lds, st, a);
+    else hipLaunchKernelGGL(k_conv_lds<2>, grid, dim3(256), lds, st, a);
+    return;
+  }
+  const int ks = conv_ksplit(np, a.co);
+  const dim3 grid(conv_blocks(np, a.co));
+  if (mode == 0) hipLaunchKernelGGL(k_conv<0>, grid, dim3(256), 0, st, a, ks);
+  else if (mode == 1) hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, st, a, ks);
+  else hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, st, a, ks);
+}
 
 // forward of nb <= B samples whose features are in s->xin
 static int forward(lbm_sur *s, int nb) {
@@ -623,11 +1095,11 @@ static int forward(lbm_sur *s, int nb) {
     if (l.last) {
       a.yf = s->out;
       a.cf = 3;
-      hipLaunchKernelGGL(k_conv<2>, dim3(conv_blocks(np, l.co)), dim3(256), 0, s->st, a);
+      launch_conv(2, a, s->st);
     } else {
       a.y = s->t[l.out].act;
       a.relu = l.relu;
-      hipLaunchKernelGGL(k_conv<0>, dim3(conv_blocks(np, l.co)), dim3(256), 0, s->st, a);
+      launch_conv(0, a, s->st);
     }
   }
   SCHK(s, hipGetLastError());
@@ -645,10 +1117,7 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s->st, (const double *)s->lpart, nbl, s->lsum);
   SCHK(s, hipGetLastError());
   if (!grads) return LBM_OK;
-  for (Tensor &T : s->t) {
-    const long long n = npix_at(s, T.level, nb) * T.c;
-    hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s->st, T.grad, n, 0.f);
-  }
+  hipLaunchKernelGGL(k_fill, dim3(nblk(s->grad_n)), dim3(256), 0, s->st, s->grad_arena, s->grad_n, 0.f);
   size_t pi = s->pool_pairs.size();
   for (int li = (int)s->L.size() - 1; li >= 0; --li) {
     const Layer &l = s->L[li];
@@ -657,38 +1126,64 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
     const long long np = (long long)nb * h * w;
     const int ci = l.ca + l.cs;
     // gradient of this layer's output, bf16 with the relu mask applied
-    const u16 *gy;
-    if (l.last) {
-      gy = s->gout;
-    } else {
-      Tensor &O = s->t[l.out];
-      // reuse the activation-gradient scratch: gout holds the bf16 copy
-      hipLaunchKernelGGL(k_mask_cast, dim3(nblk(np * l.co)), dim3(256), 0, s->st, (const float *)O.grad,
-                         (const u16 *)O.act, s->gout, np * l.co, O.relu ? 1 : 0);
-      gy = s->gout;
-    }
-    // weight and bias gradients
-    WgradArgs wa;
-    wa.gy = gy;
-    wa.a = l.a >= 0 ? s->t[l.a].act : nullptr;
-    wa.s = S.act;
-    wa.ca = l.ca;
-    wa.cs = l.cs;
-    wa.co = l.co;
-    wa.nb = nb;
-    wa.h = h;
-    wa.wd = w;
-    wa.nchunks = (int)((np + WGCHUNK - 1) / WGCHUNK);
-    wa.part = s->part;
+    // (gout holds it), and the weight and bias gradients
+    const u16 *gy = s->gout;
     const long long nw = (long long)l.co * 9 * ci;
-    const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 9 * wa.nchunks;
-    hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->st, wa);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(nw)), dim3(256), 0, s->st, (const float *)s->part, nw, wa.nchunks,
-                       s->gp + l.woff);
-    const int bchunks = (int)((np + WCHUNK - 1) / WCHUNK);
-    hipLaunchKernelGGL(k_bias_part, dim3(bchunks), dim3(256), 0, s->st, gy, np, l.co, s->part);
-    hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk(l.co)), dim3(256), 0, s->st, (const float *)s->part, (long long)l.co,
-                       bchunks, s->gp + l.boff);
+    const int nchunks = (int)((np + WGCHUNK - 1) / WGCHUNK);
+    if (wgrad_transposed(l, w)) {
+      const Tensor &O = s->t[l.last ? l.s : l.out];
+      const unsigned ptiles = (unsigned)((np + TP - 1) / TP);
+      hipLaunchKernelGGL(k_gy_prep, dim3(ptiles, l.co / 32), dim3(256), 0, s->st, l.last ? nullptr : (const float *)O.grad,
+                         l.last ? nullptr : (const u16 *)O.act, (!l.last && O.relu) ? 1 : 0,
+                         l.last ? (const u16 *)s->gout : nullptr, s->gout, s->gT, s->bpart, np, l.co);
+      hipLaunchKernelGGL(k_x_prep, dim3((unsigned)((w + 8 + TP - 1) / TP), (unsigned)(nb * (h + 2)), (unsigned)((ci + 31) / 32)),
+                         dim3(256), 0, s->st, l.a >= 0 ? (const u16 *)s->t[l.a].act : nullptr, l.ca,
+                         (const u16 *)S.act, l.cs, nb, h, w, s->xT);
+      WgradTArgs wa;
+      wa.gT = s->gT;
+      wa.xT = s->xT;
+      wa.P = np;
+      wa.co = l.co;
+      wa.ci = ci;
+      wa.nb = nb;
+      wa.h = h;
+      wa.w = w;
+      wa.cseg = wgrad_cseg(np, l.co, ci);
+      wa.nchunks = (int)((np / 8 + wa.cseg - 1) / wa.cseg);
+      wa.part = s->part;
+      const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 3 * wa.nchunks;
+      hipLaunchKernelGGL(k_wgrad_t, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->st, wa);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk32(nw)), dim3(256), 0, s->st, (const float *)s->part, nw, wa.nchunks,
+                         s->gp + l.woff);
+      hipLaunchKernelGGL(k_reduce_rows, dim3(l.co), dim3(256), 0, s->st, (const float *)s->bpart, (long long)ptiles, l.co,
+                         s->gp + l.boff);
+    } else {
+      if (!l.last) {
+        Tensor &O = s->t[l.out];
+        hipLaunchKernelGGL(k_mask_cast, dim3(nblk(np * l.co)), dim3(256), 0, s->st, (const float *)O.grad,
+                           (const u16 *)O.act, s->gout, np * l.co, O.relu ? 1 : 0);
+      }
+      WgradArgs wa;
+      wa.gy = gy;
+      wa.a = l.a >= 0 ? s->t[l.a].act : nullptr;
+      wa.s = S.act;
+      wa.ca = l.ca;
+      wa.cs = l.cs;
+      wa.co = l.co;
+      wa.nb = nb;
+      wa.h = h;
+      wa.wd = w;
+      wa.nchunks = nchunks;
+      wa.part = s->part;
+      const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 9 * nchunks;
+      hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->st, wa);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk32(nw)), dim3(256), 0, s->st, (const float *)s->part, nw, nchunks,
+                         s->gp + l.woff);
+      const int bchunks = (int)((np + WCHUNK - 1) / WCHUNK);
+      hipLaunchKernelGGL(k_bias_part, dim3(bchunks), dim3(256), 0, s->st, gy, np, l.co, s->part);
+      hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk32(l.co)), dim3(256), 0, s->st, (const float *)s->part,
+                         (long long)l.co, bchunks, s->gp + l.boff);
+    }
     // input gradient (not for the network input)
     if (li > 0) {
       ConvArgs a;
@@ -705,7 +1200,7 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
       a.g1 = S.grad;
       if (l.ca > 0)
         hipLaunchKernelGGL(k_fill, dim3(nblk(np * l.ca)), dim3(256), 0, s->st, s->gup, np * l.ca, 0.f);
-      hipLaunchKernelGGL(k_conv<1>, dim3(conv_blocks(np, ci)), dim3(256), 0, s->st, a);
+      launch_conv(1, a, s->st);
       if (l.ca > 0) {
         Tensor &A = s->t[l.a];
         hipLaunchKernelGGL(k_up_bwd, dim3(nblk(np / 4 * l.ca)), dim3(256), 0, s->st, (const float *)s->gup, A.grad,
@@ -857,6 +1352,25 @@ int lbm_sur_set_dataset(lbm_sur *s, const float *x, const float *t, const uint8_
 static int gather_batch(lbm_sur *s, const int64_t *idx, int nb, long long *nfl) {
   const long long ps = (long long)s->H * s->W;
   *nfl = 0;
+  if (nb <= MAXGATHER) {  // one launch
+    GatherArgs g;
+    for (int k = 0; k < nb; ++k) {
+      if (idx[k] < 0 || idx[k] >= s->dn) return sfail(s, LBM_EINVAL, "lbm_sur: dataset index out of range");
+      g.idx[k] = idx[k];
+      *nfl += s->dfluid[idx[k]];
+    }
+    g.ps = ps;
+    g.cin = s->cin;
+    g.dx = s->dx;
+    g.dt = s->dt;
+    g.dm = s->dm;
+    g.xin = s->xin;
+    g.tgt = s->tgt;
+    g.mask = s->mask;
+    hipLaunchKernelGGL(k_gather, dim3(256, nb), dim3(256), 0, s->st, g);
+    SCHK(s, hipGetLastError());
+    return LBM_OK;
+  }
   for (int k = 0; k < nb; ++k) {
     const long long i = idx[k];
     if (i < 0 || i >= s->dn) return sfail(s, LBM_EINVAL, "lbm_sur: dataset index out of range");
@@ -923,14 +1437,14 @@ int lbm_sur_destroy(lbm_sur *s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   for (Tensor &T : s->t) {
     if (T.act) (void)hipFree(T.act);
-    if (T.grad) (void)hipFree(T.grad);
+    T.grad = nullptr;  // in the arena
   }
   for (Layer &l : s->L) {
     if (l.wb) (void)hipFree(l.wb);
     if (l.wd) (void)hipFree(l.wd);
   }
   void *ptrs[] = {s->p, s->gp, s->m, s->v, s->out, s->tgt, s->mask, s->xin, s->gout, s->part, s->gup,
-                  s->lpart, s->lsum, s->dx, s->dt, s->dm};
+                  s->gT, s->xT, s->bpart, s->grad_arena, s->lpart, s->lsum, s->dx, s->dt, s->dm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (s->st) (void)hipStreamDestroy(s->st);

@@ -93,11 +93,16 @@ def copy_net_params(test_layer, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("test_layer", [0, 2, 5, 10, 11, 12, 14, 19, 20, 21, 22])
-def test_gpu_layer_forward_and_gradient_match_oracle(test_layer):
+@pytest.mark.parametrize("test_layer,H,W", [(l, 64, 96) for l in [0, 2, 5, 10, 11, 12, 14, 19, 20, 21, 22]]
+                         + [(l, 128, 256) for l in [0, 1, 20, 21, 22]])
+def test_gpu_layer_forward_and_gradient_match_oracle(test_layer, H, W):
+    """64x96: the split-K convolutions everywhere, the channel-major weight
+    gradient on the levels whose width is a multiple of 8 and the pixel-
+    gather one on the others; 128x256 (2 samples): the LDS-staged
+    convolutions at the full-resolution level."""
     from lbm_amd.surrogate import Surrogate
 
-    H, W, n = 64, 96, 2
+    n = 2
     s = Surrogate(batch=2, height=H, width=W, seed=3)
     p = copy_net_params(test_layer, seed=10 + test_layer)
     s.set_params(p)
@@ -105,7 +110,7 @@ def test_gpu_layer_forward_and_gradient_match_oracle(test_layer):
     x = np.abs(rng.standard_normal((n, H, W, 5))).astype(np.float32)  # copies keep positive values through relu
     t = (0.5 * rng.standard_normal((n, H, W, 3))).astype(np.float32)
     mask = (rng.random((n, H, W)) > 0.1).astype(np.uint8)
-    print(f"surrogate layer {test_layer} vs oracle", flush=True)
+    print(f"surrogate layer {test_layer} vs oracle at {H}x{W}", flush=True)
     y = s.forward(x)
     yo = SO.Net(p).forward(x)
     loss, g = s.loss_grad(x, t, mask)
