@@ -224,6 +224,16 @@ int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nont
  * src/lbm/solver.cpp:715-740. */
 int lbm_set_halo(lbm_handle *h, int32_t lds);
 
+/* LBMSolver::forcing (src/lbm/solver.cpp:626-696, configuration key
+ * "forcing"), D2Q9: after every collision (and after the first one in
+ * lbm_init_fields) the cells of the x = 0 face take
+ * f_i = feq_i(p_inlet, u_v) + f_i(v) - feq_i(rho_v, u_v) from the cell v next
+ * to the x = nx-1 face in their row, and the cells of the x = nx-1 face the
+ * same with p_outlet from the cell next to the x = 0 face; feq_i(p, u_v)
+ * takes only u_v's x component into c_i.u (as the reference does).  The
+ * pushed values then stream as the step's own.  on = 0 turns it off. */
+int lbm_set_pressure_forcing(lbm_handle *h, int32_t on, double p_inlet, double p_outlet);
+
 /* Width of the population offsets of the step kernel: 32 (default whenever a
  * population plane fits in 4 GiB; one 32-bit offset per access) or 64.
  * Both give identical results; 32 is rejected for planes above 4 GiB. */

@@ -32,8 +32,14 @@ int lbm_host_grid_centers(const lbm_host_grid *g, double *out);
 int lbm_host_grid_neighbors(const lbm_host_grid *g, int64_t *out);
 /* boundary-cell flag [n] */
 int lbm_host_grid_bndry(const lbm_host_grid *g, uint8_t *out);
-/* structured window extents nxyz[3] and structured index per cell [n] */
+/* structured window extents nxyz[3] and structured index per cell [n]
+ * (single-level grids; -1 for a multi-level grid) */
 int lbm_host_grid_structured(const lbm_host_grid *g, int64_t *nxyz, int64_t *index);
+/* tree level and parent cell (-1 on the partition level) of every cell [n]
+ * (CartesianGridBase::level / parent, src/cartesiangrid_base.h; levels
+ * above the partition level come from uniformLevel > partitionLevel or
+ * maxRfnmtLvl > uniformLevel) */
+int lbm_host_grid_levels(const lbm_host_grid *g, int32_t *level, int64_t *parent);
 /* surfaces in name order: count, then name / cell list / normal dirs of surface i */
 int32_t lbm_host_grid_surface_count(const lbm_host_grid *g);
 int lbm_host_grid_surface(const lbm_host_grid *g, int32_t i, char *name, int32_t name_len, int64_t *ncell,

@@ -51,7 +51,7 @@ class Sums(ct.Structure):
 EXPORTS = ["lbm_create", "lbm_set_bc", "lbm_init_fields", "lbm_step", "lbm_step_check", "lbm_step_ex", "lbm_step_ex_noalloc",
            "lbm_get_macros",
            "lbm_reduce", "lbm_get_populations", "lbm_set_populations", "lbm_time_steps", "lbm_time_step_groups", "lbm_set_launch",
-           "lbm_set_halo", "lbm_set_index_width", "lbm_set_storage", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
+           "lbm_set_halo", "lbm_set_pressure_forcing", "lbm_set_index_width", "lbm_set_storage", "lbm_info", "lbm_synchronize", "lbm_destroy", "lbm_last_error", "lbm_copy_bench"]
 
 _lib = None
 
@@ -82,6 +82,7 @@ def load():
     lib.lbm_set_launch.argtypes = [P, ct.c_int32, ct.c_int32, ct.c_int32]
     lib.lbm_set_index_width.argtypes = [P, ct.c_int32]
     lib.lbm_set_halo.argtypes = [P, ct.c_int32]
+    lib.lbm_set_pressure_forcing.argtypes = [P, ct.c_int32, ct.c_double, ct.c_double]
     lib.lbm_set_storage.argtypes = [P, ct.c_int32]
     lib.lbm_info.argtypes = [P, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32)]
     lib.lbm_synchronize.argtypes = [P]
@@ -231,6 +232,10 @@ class Lattice:
     def set_halo(self, lds):
         """0: gather from global memory (L2 reuse); 1: LDS-staged x halo."""
         self._check(self.lib.lbm_set_halo(self.h, int(lds)))
+
+    def set_pressure_forcing(self, on, p_inlet=1.0, p_outlet=1.0):
+        """The reference's 'forcing' correction on the x faces (D2Q9)."""
+        self._check(self.lib.lbm_set_pressure_forcing(self.h, int(on), float(p_inlet), float(p_outlet)))
 
     def set_index_width(self, bits):
         self._check(self.lib.lbm_set_index_width(self.h, int(bits)))

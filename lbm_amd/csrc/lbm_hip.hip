@@ -248,12 +248,66 @@ struct lbm_handle {
   int bx = 256, by = 1, nt = 1;  // nontemporal loads/stores measured fastest on gfx950
   bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
   bool lds = false;              // LDS-staged x halo (lbm_set_halo)
+  bool forcing = false;          // lbm_set_pressure_forcing
+  double p_in = 1.0, p_out = 1.0;
+  double *fmac = nullptr;        // (rho, u_x, u_y) of the forced cells before the correction
   long long steps = 0;
   int device = 0;  // device current at lbm_create
   std::string err;
 };
 
 static thread_local std::string g_err;
+
+// LBMSolver::forcing (src/lbm/solver.cpp:626-696) on the D2Q9 post-collision
+// state in buf: thread 2y + s rewrites cell (x_f, y) of the outlet face
+// (s = 0, x_f = nx-1, from v = (1, y), p_out) or the inlet face (s = 1,
+// x_f = 0, from v = (nx-2, y), p_in) as f_i = feq_i(p, u_v) + f_i(v) -
+// feq_i(rho_v, u_v), feq_i(p, u_v) taking only u_v's x component into c_i.u.
+// rho_v and u_v are the moments of v's post-collision populations, which the
+// BGK collision (no body force) leaves equal to those of the step.
+// Populations are stored shifted by the weights.  The moments of the forced
+// cell's own post-collision populations (those of its step) go to
+// mac[t][3] first: lbm_get_macros reports them for these cells.
+template <typename ST>
+__global__ void __launch_bounds__(256) k_forcing(ST *buf, long long plane, int px, int nx, int ny, double p_in,
+                                                 double p_out, double *mac) {
+  using L = lbm::Lat<9>;
+  using C = double;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2 * ny) return;
+  const int y = t >> 1, side = t & 1;
+  const int vx = side ? nx - 2 : 1, fx = side ? 0 : nx - 1;
+  const C p = side ? p_in : p_out;
+  const long long vc = vx + (long long)px * y, fc = fx + (long long)px * y;
+  C f[9], rho = 0, jx = 0, jy = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    f[j] = C(L::w[j]) + C(lbm::Store<ST>::ld(buf + j * plane + vc));
+    rho += f[j];
+    jx += C(L::cx[j]) * f[j];
+    jy += C(L::cy[j]) * f[j];
+  }
+  const C ux = jx / rho, uy = jy / rho, vsq = ux * ux + uy * uy;
+  C fr = 0, fjx = 0, fjy = 0;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const C g = C(L::w[j]) + C(lbm::Store<ST>::ld(buf + j * plane + fc));
+    fr += g;
+    fjx += C(L::cx[j]) * g;
+    fjy += C(L::cy[j]) * g;
+  }
+  mac[3 * t] = fr;
+  mac[3 * t + 1] = fjx / fr;
+  mac[3 * t + 2] = fjy / fr;
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    const C w = C(L::w[j]);
+    const C cu = C(L::cx[j]) * ux + C(L::cy[j]) * uy, cux = C(L::cx[j]) * ux;
+    const C feqv = w * rho * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * vsq);
+    const C feqp = w * p * (C(1) + C(3) * cux + C(4.5) * cux * cux - C(1.5) * vsq);
+    lbm::Store<ST>::st(buf + j * plane + fc, feqp + f[j] - feqv - w);
+  }
+}
 
 static int fail(lbm_handle *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -618,6 +672,8 @@ int lbm_set_bc(lbm_handle *h, const lbm_bc_desc *bcs, int32_t n) {
   return build_links(h);
 }
 
+static int apply_forcing(lbm_handle *h);
+
 int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   if (!h || !rho || !u) return fail(h, LBM_EINVAL, "lbm_init_fields: null argument");
   int rc = ensure_io(h, (size_t)h->ncell * (1 + h->dim) * sizeof(double));
@@ -635,6 +691,10 @@ int lbm_init_fields(lbm_handle *h, const double *rho, const double *u) {
   launch_dims(h, g, b);
   dispatch_qs<InitL>(h->Q, h->storage, a, g, b, h->stream, (const double *)rd, (const double *)ud);
   HIPCHK(h, hipGetLastError());
+  if (h->forcing) {  // the initial condition ends with the first collision
+    rc = apply_forcing(h);
+    if (rc) return rc;
+  }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   h->steps = 0;
   return LBM_OK;
@@ -660,6 +720,42 @@ static int ensure_snapshots(lbm_handle *h) {
 
 // partials: where a check step puts its per-block sums (the handle's buffer
 // unless given)
+// (rho, u) of the forced cells as their step left them, into the macro
+// fields k_macros wrote (compact x-fastest order)
+__global__ void __launch_bounds__(256) k_forcing_macros(const double *mac, int nx, int ny, double *rho, double *u) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= 2 * ny) return;
+  const int y = t >> 1, fx = (t & 1) ? 0 : nx - 1;
+  const long long cc = fx + (long long)nx * y;
+  rho[cc] = mac[3 * t];
+  u[2 * cc] = mac[3 * t + 1];
+  u[2 * cc + 1] = mac[3 * t + 2];
+}
+
+// the forcing correction on the state the last launch wrote
+static int apply_forcing(lbm_handle *h) {
+  const dim3 g((unsigned)((2 * h->ny + 255) / 256)), b(256);
+  void *buf = h->buf[h->cur];
+  switch (h->storage) {
+    case LBM_STORE_F32:
+      hipLaunchKernelGGL(k_forcing<float>, g, b, 0, h->stream, (float *)buf, h->plane, h->px, h->nx, h->ny, h->p_in, h->p_out, h->fmac);
+      break;
+    case LBM_STORE_F64:
+      hipLaunchKernelGGL(k_forcing<double>, g, b, 0, h->stream, (double *)buf, h->plane, h->px, h->nx, h->ny, h->p_in,
+                         h->p_out, h->fmac);
+      break;
+    case LBM_STORE_F16:
+      hipLaunchKernelGGL(k_forcing<__half>, g, b, 0, h->stream, (__half *)buf, h->plane, h->px, h->nx, h->ny, h->p_in,
+                         h->p_out, h->fmac);
+      break;
+    default:
+      hipLaunchKernelGGL(k_forcing<__hip_bfloat16>, g, b, 0, h->stream, (__hip_bfloat16 *)buf, h->plane, h->px, h->nx,
+                         h->ny, h->p_in, h->p_out, h->fmac);
+  }
+  HIPCHK(h, hipGetLastError());
+  return LBM_OK;
+}
+
 static int launch_step(lbm_handle *h, int mode, double *partials = nullptr) {
   KArgs a = make_args(h);
   if (mode != 0) {
@@ -679,6 +775,7 @@ static int launch_step(lbm_handle *h, int mode, double *partials = nullptr) {
   if (mode == 1 || mode == 3) h->snap_last ^= 1;
   h->cur ^= 1;
   h->steps++;
+  if (h->forcing) return apply_forcing(h);
   return LBM_OK;
 }
 
@@ -776,6 +873,11 @@ int lbm_get_macros(lbm_handle *h, double *rho, double *u) {
   launch_dims(h, g, b);
   dispatch_qs<MacroL>(h->Q, h->storage, a, g, b, h->stream, rd, ud);
   HIPCHK(h, hipGetLastError());
+  if (h->forcing) {
+    hipLaunchKernelGGL(k_forcing_macros, dim3((unsigned)((2 * h->ny + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double *)h->fmac, h->nx, h->ny, rd, ud);
+    HIPCHK(h, hipGetLastError());
+  }
   HIPCHK(h, hipMemcpyAsync(rho, rd, h->ncell * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipMemcpyAsync(u, ud, h->ncell * h->dim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -931,6 +1033,26 @@ int lbm_set_halo(lbm_handle *h, int32_t lds) {
   return LBM_OK;
 }
 
+int lbm_set_pressure_forcing(lbm_handle *h, int32_t on, double p_inlet, double p_outlet) {
+  if (!h || (on != 0 && on != 1)) return fail(h, LBM_EINVAL, "lbm_set_pressure_forcing: on must be 0 or 1");
+  if (on) {
+    if (h->Q != 9) return fail(h, LBM_EINVAL, "lbm_set_pressure_forcing: D2Q9 only (the reference's forcing is 2D)");
+    if (h->nx < 4) return fail(h, LBM_EINVAL, "lbm_set_pressure_forcing: nx must be at least 4");
+    if (h->F[0] != 0 || h->F[1] != 0 || h->F[2] != 0)
+      return fail(h, LBM_EINVAL, "lbm_set_pressure_forcing: not with a body force (the correction reads the moments "
+                                 "the collision keeps)");
+    if (!(p_inlet > 0) || !(p_outlet > 0)) return fail(h, LBM_EINVAL, "lbm_set_pressure_forcing: pressures must be > 0");
+    if (!h->fmac) {
+      HIPCHK(h, hipMalloc(&h->fmac, (size_t)2 * h->ny * 3 * sizeof(double)));
+      HIPCHK(h, hipMemset(h->fmac, 0, (size_t)2 * h->ny * 3 * sizeof(double)));
+    }
+  }
+  h->forcing = on != 0;
+  h->p_in = p_inlet;
+  h->p_out = p_outlet;
+  return LBM_OK;
+}
+
 int lbm_set_index_width(lbm_handle *h, int32_t bits) {
   if (!h) return fail(h, LBM_EINVAL, "lbm_set_index_width: null handle");
   if (bits == 64) {
@@ -965,6 +1087,7 @@ int lbm_destroy(lbm_handle *h) {
   for (int b = 0; b < 2; ++b)
     if (h->buf[b]) hipFree(h->buf[b]);
   if (h->flags) hipFree(h->flags);
+  if (h->fmac) hipFree(h->fmac);
   if (h->links) hipFree(h->links);
   for (int b = 0; b < 2; ++b)
     if (h->snap[b]) hipFree(h->snap[b]);

@@ -58,6 +58,28 @@ auto dirVec(GInt dim, GInt dir) -> std::array<GDouble, 3> {
   return v;
 }
 
+// state of the tree refinement: the generator's boundary flag, children,
+// flood-fill flags and the id range of each level
+struct CartesianGrid::Refinement {
+  std::vector<uint8_t> gb, inside, marked;
+  std::vector<std::array<GInt, 8>> child;
+  std::vector<GInt> begin, end;
+};
+
+namespace {
+// include/common/math/cartesian.h:72-153: child directions, the sibling that
+// neighbours a child in a direction, and the child of the parent's
+// neighbour that does
+constexpr int kChildDir[8][3] = {{-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {1, 1, -1},
+                                 {-1, -1, 1},  {1, -1, 1},  {-1, 1, 1},  {1, 1, 1}};
+constexpr int kNghbrInside[8][6] = {{-1, 1, -1, 2, -1, 4}, {0, -1, -1, 3, -1, 5}, {-1, 3, 0, -1, -1, 6},
+                                    {2, -1, 1, -1, -1, 7}, {-1, 5, -1, 6, 0, -1}, {4, -1, -1, 7, 1, -1},
+                                    {-1, 7, 4, -1, 2, -1}, {6, -1, 5, -1, 3, -1}};
+constexpr int kNghbrParentChild[8][6] = {{1, -1, 2, -1, 4, -1}, {-1, 0, 3, -1, 5, -1}, {3, -1, -1, 0, 6, -1},
+                                         {-1, 2, -1, 1, 7, -1}, {5, -1, 6, -1, -1, 0}, {-1, 4, 7, -1, -1, 1},
+                                         {7, -1, -1, 4, -1, 2}, {-1, 6, -1, 5, -1, 3}};
+}  // namespace
+
 void CartesianGrid::generate(Configuration& cfg) {
   m_dim = (int)cfg.required_config_value<GInt>("dim");
   if (m_dim < 1 || m_dim > 3) TERMM(-1, "Only 1D, 2D and 3D grids are supported");
@@ -67,10 +89,11 @@ void CartesianGrid::generate(Configuration& cfg) {
   if (partitionLvl > uniformLvl) TERMM(-1, "Invalid definition of grid level partitionLevel >= uniformLevel");
   const GInt maxRfnmt = cfg.opt_config_value<GInt>("maxRfnmtLvl", uniformLvl);
   if (maxRfnmt < uniformLvl) TERMM(-1, "Invalid definition of grid level uniformLevel >= maxRfnmtLvl");
-  if (partitionLvl != uniformLvl || uniformLvl != maxRfnmt)
-    TERMM(-1, "Multi-level grids are not supported: partitionLevel, uniformLevel and maxRfnmtLvl must be equal");
   if (partitionLvl < 1 || partitionLvl > 14) TERMM(-1, "partitionLevel must be in 1..14");
+  if (maxRfnmt > 16) TERMM(-1, "maxRfnmtLvl must be at most 16");
   m_level = partitionLvl;
+  m_maxLvl = maxRfnmt;
+  m_uniformLvl = uniformLvl;
   cfg.opt_config_value<bool>("dry-run", false);
   cfg.opt_config_value<std::string>("outputDir", "out");
   cfg.opt_config_value<std::string>("gridFileName", "grid");
@@ -100,9 +123,9 @@ void CartesianGrid::generate(Configuration& cfg) {
     decisive = ext[d] > ext[decisive] ? d : decisive;
     m_cog[d] = m_bbox.min[d] + 0.5 * (m_bbox.max[d] - m_bbox.min[d]);
   }
-  m_len.assign(m_level + 1, 0.0);
+  m_len.assign(m_maxLvl + 1, 0.0);
   m_len[0] = (1.0 + 1.0 / std::pow(2.0, (GDouble)MAX_LVL)) * ext[decisive];
-  for (GInt l = 1; l <= m_level; ++l) m_len[l] = 0.5 * m_len[l - 1];
+  for (GInt l = 1; l <= m_maxLvl; ++l) m_len[l] = 0.5 * m_len[l - 1];
 
   // per-axis centre coordinates for each level, parent + (len/2)*(+-1)
   std::vector<std::vector<std::array<GDouble, 3>>> cen(m_level + 1);
@@ -262,8 +285,69 @@ void CartesianGrid::generate(Configuration& cfg) {
     m_cell[q] = cells[order[q]];
     centreOf(m_level, m_cell[q][0], m_cell[q][1], m_cell[q][2], &m_center[q * 3]);
   }
+  // main neighbours of the partition level
+  std::vector<GInt> at(w.size(), INVALID_CELLID);
+  for (GInt q = 0; q < N; ++q) at[w.idx(m_cell[q][0], m_cell[q][1], m_cell[q][2])] = q;
+  m_stride = 2 * m_dim;
+  m_nghbr.assign(N * m_stride, INVALID_CELLID);
+  for (GInt q = 0; q < N; ++q) {
+    for (int dir = 0; dir < 2 * m_dim; ++dir) {
+      std::array<GInt, 3> co = m_cell[q];
+      co[dir / 2] += (dir % 2) ? 1 : -1;
+      if (!w.inside(co[0], co[1], co[2])) continue;
+      m_nghbr[q * m_stride + dir] = at[w.idx(co[0], co[1], co[2])];
+    }
+  }
+  m_cellLvl.assign(N, m_level);
+  m_parent.assign(N, INVALID_CELLID);
+  // finer levels
+  if (m_maxLvl > m_level) {
+    Refinement st;
+    st.gb.resize(N);
+    for (GInt q = 0; q < N; ++q) st.gb[q] = bnd[w.idx(m_cell[q][0], m_cell[q][1], m_cell[q][2])];
+    st.child.assign(N, {{INVALID_CELLID, INVALID_CELLID, INVALID_CELLID, INVALID_CELLID, INVALID_CELLID, INVALID_CELLID,
+                         INVALID_CELLID, INVALID_CELLID}});
+    st.begin.assign(m_maxLvl + 1, 0);
+    st.end.assign(m_maxLvl + 1, 0);
+    st.end[m_level] = N;
+    for (GInt l = m_level; l < uniformLvl; ++l) refineLevel(l, false, maxNoCells, st);
+    for (GInt l = uniformLvl; l < m_maxLvl; ++l) refineLevel(l, true, maxNoCells, st);
+    m_leaf.assign(size(), 1);
+    for (GInt q = 0; q < size(); ++q)
+      for (GInt c : st.child[q])
+        if (c != INVALID_CELLID) m_leaf[q] = 0;
+  } else {
+    m_leaf.assign(N, 1);
+  }
+  m_levels.clear();
+  for (GInt q = 0; q < size(); ++q)
+    if (m_levels.empty() || m_levels.back() != m_cellLvl[q]) m_levels.push_back(m_cellLvl[q]);
   if (align) alignWithSurface(alignDir);
-  // structured window and main neighbours
+  if (m_levels.size() == 1) {
+    setWindow();
+  } else {  // no single structured window: see levelGrid
+    m_n = {{0, 0, 0}};
+    m_struct.clear();
+    m_cellOf.clear();
+  }
+}
+
+// structured window of a single-level grid: the bounding range of the level
+// indices, each cell's index in it and the cell (or none) at each position
+void CartesianGrid::setWindow() {
+  const GInt N = size();
+  std::array<GInt, 3> mn{{std::numeric_limits<GInt>::max(), std::numeric_limits<GInt>::max(), std::numeric_limits<GInt>::max()}};
+  std::array<GInt, 3> mx{{std::numeric_limits<GInt>::min(), std::numeric_limits<GInt>::min(), std::numeric_limits<GInt>::min()}};
+  for (GInt q = 0; q < N; ++q)
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = std::min(mn[d], m_cell[q][d]);
+      mx[d] = std::max(mx[d], m_cell[q][d]);
+    }
+  Window w;
+  for (int d = 0; d < 3; ++d) {
+    w.lo[d] = mn[d];
+    w.n[d] = mx[d] - mn[d] + 1;
+  }
   m_lo = w.lo;
   m_n = w.n;
   m_struct.assign(N, -1);
@@ -273,16 +357,137 @@ void CartesianGrid::generate(Configuration& cfg) {
     m_struct[q] = s;
     m_cellOf[s] = q;
   }
-  m_stride = 2 * m_dim;
-  m_nghbr.assign(N * m_stride, INVALID_CELLID);
-  for (GInt q = 0; q < N; ++q) {
-    for (int dir = 0; dir < 2 * m_dim; ++dir) {
-      std::array<GInt, 3> co = m_cell[q];
-      co[dir / 2] += (dir % 2) ? 1 : -1;
-      if (!w.inside(co[0], co[1], co[2])) continue;
-      m_nghbr[q * m_stride + dir] = m_cellOf[w.idx(co[0], co[1], co[2])];
+}
+
+// refineGrid for level lvl (src/gridgenerator/cartesiangrid_generation.h:343-417):
+// the children of every cell of the level (or of its boundary cells) are
+// appended in parent order (refineCell, :419-451), get their neighbours
+// from their siblings and the parent's neighbours (findChildLevelNghbrs,
+// :453-506), and the new level's outside cells are deleted: flood fill in
+// cell order from the non-boundary cells, then deletion from the end with
+// the level's last cell copied into each hole (:509-661)
+void CartesianGrid::refineLevel(GInt lvl, bool onlyBndry, GInt maxNoCells, Refinement& st) {
+  const int nch = 1 << m_dim, nn = 2 * m_dim;
+  const GInt b = st.begin[lvl], e = st.end[lvl];
+  std::vector<GInt> parents;
+  for (GInt c = b; c < e; ++c)
+    if (!onlyBndry || st.gb[c]) parents.push_back(c);
+  const GInt nb = size(), ne = nb + (GInt)parents.size() * nch;
+  if (ne > maxNoCells) TERMM(-1, "Out of memory! Increase maxNoCells");
+  m_cell.resize(ne);
+  m_center.resize(ne * 3, 0.0);
+  m_cellLvl.resize(ne);
+  m_parent.resize(ne);
+  m_nghbr.resize(ne * m_stride, INVALID_CELLID);
+  st.gb.resize(ne);
+  st.inside.assign(ne, 0);
+  st.marked.assign(ne, 0);
+  st.child.resize(ne);
+  const GDouble len = m_len[lvl + 1];
+  for (size_t k = 0; k < parents.size(); ++k) {
+    const GInt p = parents[k];
+    for (int ch = 0; ch < nch; ++ch) {
+      const GInt c = nb + (GInt)k * nch + ch;
+      m_cellLvl[c] = lvl + 1;
+      m_parent[c] = p;
+      for (int d = 0; d < 3; ++d) m_cell[c][d] = d < m_dim ? 2 * m_cell[p][d] + (kChildDir[ch][d] > 0 ? 1 : 0) : 0;
+      for (int d = 0; d < 3; ++d) m_center[c * 3 + d] = d < m_dim ? m_center[p * 3 + d] + 0.5 * len * kChildDir[ch][d] : 0.0;
+      st.child[c].fill(INVALID_CELLID);
+      for (int dir = 0; dir < nn; ++dir) m_nghbr[c * m_stride + dir] = INVALID_CELLID;
+      st.gb[c] = st.gb[p] ? (m_geom->cutWithCell(&m_center[c * 3], len) ? 1 : 0) : 0;
+      st.child[p][ch] = c;
     }
   }
+  for (GInt p = b; p < e; ++p)
+    for (int ch = 0; ch < nch; ++ch) {
+      const GInt c = st.child[p][ch];
+      if (c == INVALID_CELLID) continue;
+      for (int dir = 0; dir < nn; ++dir) {
+        GInt& nbr = m_nghbr[c * m_stride + dir];
+        if (nbr != INVALID_CELLID) continue;
+        const int ni = kNghbrInside[ch][dir];
+        if (ni != -1) {
+          nbr = st.child[p][ni];
+        } else {
+          const int pc = kNghbrParentChild[ch][dir];
+          const GInt pn = m_nghbr[p * m_stride + dir];
+          if (pn != INVALID_CELLID && pc != -1 && st.child[pn][pc] != INVALID_CELLID) nbr = st.child[pn][pc];
+        }
+      }
+    }
+  // markOutsideCells / floodCells
+  std::vector<GInt> stack;
+  for (GInt c = nb; c < ne; ++c) {
+    if (st.marked[c]) continue;
+    st.marked[c] = 1;
+    const bool isB = st.gb[c] != 0;
+    st.inside[c] = (isB || m_geom->pointIsInside(&m_center[c * 3])) ? 1 : 0;
+    if (isB) continue;
+    const uint8_t val = st.inside[c];
+    stack.assign(1, c);
+    while (!stack.empty()) {
+      const GInt cur = stack.back();
+      stack.pop_back();
+      for (int dir = 0; dir < nn; ++dir) {
+        const GInt n = m_nghbr[cur * m_stride + dir];
+        if (n == INVALID_CELLID || st.marked[n]) continue;
+        st.marked[n] = 1;
+        if (!st.gb[n]) {
+          st.inside[n] = val;
+          stack.push_back(n);
+        } else {
+          st.inside[n] = 1;
+        }
+      }
+    }
+  }
+  // deleteCell / copyCell / updateParent
+  auto updateParent = [&](GInt p, GInt from, GInt to) {
+    for (auto& x : st.child[p])
+      if (x == from) {
+        x = to;
+        return;
+      }
+  };
+  GInt end = ne;
+  for (GInt c = ne - 1; c >= nb; --c) {
+    if (st.inside[c]) continue;
+    updateParent(m_parent[c], c, INVALID_CELLID);
+    for (int dir = 0; dir < nn; ++dir) {
+      const GInt n = m_nghbr[c * m_stride + dir];
+      if (n != INVALID_CELLID) m_nghbr[n * m_stride + (dir ^ 1)] = INVALID_CELLID;
+    }
+    const GInt from = end - 1;
+    if (c != from) {
+      m_cell[c] = m_cell[from];
+      for (int d = 0; d < 3; ++d) m_center[c * 3 + d] = m_center[from * 3 + d];
+      m_cellLvl[c] = m_cellLvl[from];
+      m_parent[c] = m_parent[from];
+      st.gb[c] = st.gb[from];
+      st.inside[c] = st.inside[from];
+      st.marked[c] = st.marked[from];
+      st.child[c] = st.child[from];
+      for (int dir = 0; dir < nn; ++dir) {
+        const GInt n = m_nghbr[from * m_stride + dir];
+        m_nghbr[c * m_stride + dir] = n;
+        if (n != INVALID_CELLID) m_nghbr[n * m_stride + (dir ^ 1)] = c;
+        m_nghbr[from * m_stride + dir] = INVALID_CELLID;
+      }
+      if (m_parent[c] != INVALID_CELLID) updateParent(m_parent[c], from, c);
+      for (GInt x : st.child[c])
+        if (x != INVALID_CELLID) m_parent[x] = c;
+    }
+    --end;
+  }
+  m_cell.resize(end);
+  m_center.resize(end * 3);
+  m_cellLvl.resize(end);
+  m_parent.resize(end);
+  m_nghbr.resize(end * m_stride);
+  st.gb.resize(end);
+  st.child.resize(end);
+  st.begin[lvl + 1] = nb;
+  st.end[lvl + 1] = end;
 }
 
 // transformMaxRfnmtLvlToExtent (src/gridgenerator/cartesiangrid_generation.h:256-304):
@@ -294,12 +499,13 @@ void CartesianGrid::generate(Configuration& cfg) {
 void CartesianGrid::alignWithSurface(GInt alignDir) {
   const GInt N = (GInt)m_cell.size();
   if (alignDir < 0 || alignDir >= m_dim) TERMM(-1, "Invalid alignDir");
+  auto onMax = [&](GInt q) { return m_cellLvl[q] == m_maxLvl; };
   std::array<GDouble, 3> mn{{std::numeric_limits<GDouble>::max(), std::numeric_limits<GDouble>::max(),
                              std::numeric_limits<GDouble>::max()}};
   std::array<GDouble, 3> mx{{std::numeric_limits<GDouble>::min(), std::numeric_limits<GDouble>::min(),
                              std::numeric_limits<GDouble>::min()}};
   for (GInt q = 0; q < N; ++q)
-    for (int d = 0; d < m_dim; ++d) {
+    for (int d = 0; d < m_dim && onMax(q); ++d) {
       if (mn[d] > m_center[q * 3 + d]) mn[d] = m_center[q * 3 + d];
       if (mx[d] < m_center[q * 3 + d]) mx[d] = m_center[q * 3 + d];
     }
@@ -313,33 +519,39 @@ void CartesianGrid::alignWithSurface(GInt alignDir) {
       }
   const GDouble T = m_bbox.max[alignDir] / (mx[alignDir] - mn[alignDir]);
   for (GInt q = 0; q < N; ++q)
-    for (int d = 0; d < m_dim; ++d) {
+    for (int d = 0; d < m_dim && onMax(q); ++d) {
       GDouble& c = m_center[q * 3 + d];
       if (d == alignDir || same) c = c * T - (T * mx[d] - m_bbox.max[d]);
       else c = c * T;
     }
-  m_len[m_level] *= T;
+  m_len[m_maxLvl] *= T;
   for (GInt q = 0; q < N; ++q) {
+    if (!onMax(q)) continue;
     const GDouble* c = &m_center[q * 3];
-    if (!m_geom->cutWithCell(c, m_len[m_level]) && !m_geom->pointIsInside(c))
+    if (!m_geom->cutWithCell(c, m_len[m_maxLvl]) && !m_geom->pointIsInside(c))
       TERMM(-1, "alignNodesWithSurface moves cells out of the geometry; deleting them is not supported by this build");
   }
 }
 
 void CartesianGrid::loadGridInplace(Configuration& solverCfg) {
   const GInt N = size();
-  // determineBoundaryCells: cut and missing at least one main neighbour
+  // determineBoundaryCells: a partition-level cell or a child of a boundary
+  // cell (parents precede their children) that the geometry cuts at its
+  // level's length and that misses at least one main neighbour
   m_bndry.assign(N, 0);
   m_noBndCells = 0;
   for (GInt q = 0; q < N; ++q) {
-    bool b = m_geom->cutWithCell(&m_center[q * 3], m_len[m_level]);
-    if (b) {
-      GInt nn = 0;
-      for (int dir = 0; dir < 2 * m_dim; ++dir) nn += neighbor(q, dir) != INVALID_CELLID;
-      if (nn == 2 * m_dim) b = false;
+    bool b = false;
+    if (m_parent[q] == INVALID_CELLID || m_bndry[m_parent[q]]) {
+      b = m_geom->cutWithCell(&m_center[q * 3], cellLength(q));
+      if (b) {
+        GInt nn = 0;
+        for (int dir = 0; dir < 2 * m_dim; ++dir) nn += neighbor(q, dir) != INVALID_CELLID;
+        if (nn == 2 * m_dim) b = false;
+      }
     }
     m_bndry[q] = b;
-    m_noBndCells += b;
+    m_noBndCells += b && m_leaf[q];
   }
   // identifyBndrySurfaces
   json bndryConfig = solverCfg.getObject("boundary");
@@ -357,7 +569,7 @@ void CartesianGrid::loadGridInplace(Configuration& solverCfg) {
         for (GInt q = 0; q < N; ++q) {
           if (!m_bndry[q]) continue;
           if (neighbor(q, dir) != INVALID_CELLID) continue;
-          if (!m_geom->cutWithCell(surfName, &m_center[q * 3], m_len[m_level])) continue;
+          if (!m_geom->cutWithCell(surfName, &m_center[q * 3], cellLength(q))) continue;
           if (assigned.insert({q, dir}).second) m_surfaces.at(nameAp).addCell(q, dir);
         }
       }
@@ -457,7 +669,7 @@ void CartesianGrid::addDiagonalNghbrs() {
 // face is linked to the high-face cell of its row and vice versa.
 void CartesianGrid::checkPeriodicWrap() {
   for (int a = 0; a < 3; ++a) m_periodicAxis[a] = false;
-  if (!m_hasPeriodicLinks) return;
+  if (!m_hasPeriodicLinks || m_levels.size() != 1) return;
   for (int a = 0; a < m_dim; ++a) {
     bool any = false, all = true;
     for (GInt q = 0; q < size() && all; ++q) {
@@ -474,6 +686,61 @@ void CartesianGrid::checkPeriodicWrap() {
     }
     m_periodicAxis[a] = any && all;
   }
+}
+
+auto CartesianGrid::levelGrid(GInt lvl, std::vector<GInt>& ids) const -> std::unique_ptr<CartesianGrid> {
+  auto g = std::make_unique<CartesianGrid>();
+  g->m_dim = m_dim;
+  g->m_level = lvl;
+  g->m_maxLvl = lvl;
+  g->m_uniformLvl = lvl;
+  g->m_bbox = m_bbox;
+  g->m_cog = m_cog;
+  g->m_len = m_len;
+  g->m_geom = m_geom;
+  ids.clear();
+  std::vector<GInt> loc(size(), INVALID_CELLID);
+  for (GInt q = 0; q < size(); ++q)
+    if (m_cellLvl[q] == lvl) {
+      loc[q] = (GInt)ids.size();
+      ids.push_back(q);
+    }
+  const GInt n = (GInt)ids.size();
+  if (n == 0) TERMM(-1, "levelGrid: no cells on level " + std::to_string(lvl));
+  g->m_stride = m_stride;
+  g->m_nghbr.assign(n * m_stride, INVALID_CELLID);
+  g->m_cell.resize(n);
+  g->m_center.resize(n * 3);
+  g->m_cellLvl.assign(n, lvl);
+  g->m_parent.assign(n, INVALID_CELLID);
+  g->m_leaf.resize(n);
+  g->m_bndry.resize(n);
+  g->m_levels = {lvl};
+  for (GInt q = 0; q < n; ++q) {
+    const GInt c = ids[q];
+    g->m_cell[q] = m_cell[c];
+    for (int d = 0; d < 3; ++d) g->m_center[q * 3 + d] = m_center[c * 3 + d];
+    g->m_leaf[q] = m_leaf[c];
+    g->m_bndry[q] = m_bndry[c];
+    g->m_noBndCells += m_bndry[c] && m_leaf[c];
+    for (GInt dir = 0; dir < m_stride; ++dir) {
+      const GInt nb = m_nghbr[c * m_stride + dir];
+      if (nb == INVALID_CELLID) continue;
+      if (loc[nb] == INVALID_CELLID) TERMM(-1, "levelGrid: a neighbour link crosses levels");
+      g->m_nghbr[q * m_stride + dir] = loc[nb];
+    }
+    if (m_periodicCell.count(c)) g->m_periodicCell.insert(q);
+  }
+  for (const auto& [name, srf] : m_surfaces) {
+    Surface s(g.get());
+    for (GInt c : srf.getCellList())
+      if (loc[c] != INVALID_CELLID) s.addCell(loc[c], srf.dirOf(c));
+    g->m_surfaces.emplace(name, std::move(s));
+  }
+  g->m_hasPeriodicLinks = m_hasPeriodicLinks;
+  g->setWindow();
+  g->checkPeriodicWrap();
+  return g;
 }
 
 }  // namespace lbmh

@@ -2,8 +2,7 @@
 // order, neighbour tables, boundary cells and surfaces, and the structured
 // view the device kernels use.
 //
-// Reference behaviour restated (single-level grids, partitionLevel ==
-// uniformLevel == maxRfnmtLvl, the only form every reference LBM test uses):
+// Reference behaviour restated:
 //  - root cube: centre = centre of the geometry bounding box, length =
 //    (1 + 2^-MAX_LVL) * largest extent, halved per level
 //    (src/cartesiangrid_base.h:125-143);
@@ -15,8 +14,18 @@
 //    after each refinement (cartesiangrid_generation.h:509-552,578-620);
 //  - cell ids = ascending Hilbert index of the centre normalised by the root
 //    cube at the partition level (cartesiangrid_generation.h:664-735);
-//  - solver side: boundary cell = cut and missing a main neighbour
-//    (src/cartesiangrid.h:510-546), surfaces per configured
+//  - multi-level grids (partitionLevel < uniformLevel <= maxRfnmtLvl): the
+//    finer levels are appended after the partition level, every cell of a
+//    level refined (uniformRefineGrid) or only its boundary cells
+//    (markBndryCells / refineMarkedCells), children in parent order, their
+//    neighbours from the parent's (findChildLevelNghbrs), outside cells
+//    deleted with the reference's flood fill and copy-last compaction
+//    (cartesiangrid_generation.h:151-206,343-417,419-661); every level stays
+//    in the grid, linked only within itself, so a solver steps each level as
+//    its own lattice;
+//  - solver side: boundary cell = cut at its level's length and missing a
+//    main neighbour, tested for partition-level cells and children of
+//    boundary cells (src/cartesiangrid.h:510-546), surfaces per configured
 //    (geometry, face) (src/cartesiangrid.h:553-603), periodic links between
 //    surfaces (src/cartesiangrid.h:608-706), diagonal neighbours composed from
 //    main neighbours (src/cartesiangrid.h:451-493).
@@ -90,8 +99,10 @@ class CartesianGrid : public GridInterface {
   // copies the tables (like loadGridInplace copies the generator's grid) and
   // points the surfaces at the copy
   CartesianGrid(const CartesianGrid& o)
-      : m_dim(o.m_dim), m_level(o.m_level), m_bbox(o.m_bbox), m_cog(o.m_cog), m_len(o.m_len), m_geom(o.m_geom),
-        m_cell(o.m_cell), m_center(o.m_center), m_nghbr(o.m_nghbr), m_stride(o.m_stride), m_bndry(o.m_bndry),
+      : m_dim(o.m_dim), m_level(o.m_level), m_maxLvl(o.m_maxLvl), m_uniformLvl(o.m_uniformLvl), m_bbox(o.m_bbox),
+        m_cog(o.m_cog), m_len(o.m_len),
+        m_geom(o.m_geom), m_cell(o.m_cell), m_center(o.m_center), m_cellLvl(o.m_cellLvl), m_parent(o.m_parent),
+        m_leaf(o.m_leaf), m_levels(o.m_levels), m_nghbr(o.m_nghbr), m_stride(o.m_stride), m_bndry(o.m_bndry),
         m_noBndCells(o.m_noBndCells), m_surfaces(o.m_surfaces), m_periodicCell(o.m_periodicCell), m_lo(o.m_lo),
         m_n(o.m_n), m_struct(o.m_struct), m_cellOf(o.m_cellOf), m_periodicAxis(o.m_periodicAxis),
         m_hasPeriodicLinks(o.m_hasPeriodicLinks) {
@@ -115,13 +126,22 @@ class CartesianGrid : public GridInterface {
   }
   [[nodiscard]] auto property(GInt cellId, CellProperties p) const -> bool override {
     if (p == CellProperties::bndry) return m_bndry[cellId] != 0;
-    if (p == CellProperties::leaf) return true;
+    if (p == CellProperties::leaf) return m_leaf[cellId] != 0;
     return m_periodicCell.count(cellId) > 0;
   }
-  [[nodiscard]] auto cellLength(GInt /*cellId*/) const -> GDouble override { return m_len[m_level]; }
+  [[nodiscard]] auto cellLength(GInt cellId) const -> GDouble override { return m_len[m_cellLvl[cellId]]; }
   [[nodiscard]] auto lengthOnLvl(GInt lvl) const -> GDouble override { return m_len.at(lvl); }
   [[nodiscard]] auto partitionLvl() const -> GInt override { return m_level; }
-  [[nodiscard]] auto maxLvl() const -> GInt override { return m_level; }
+  [[nodiscard]] auto maxLvl() const -> GInt override { return m_maxLvl; }
+  [[nodiscard]] auto uniformLvl() const -> GInt { return m_uniformLvl; }
+  [[nodiscard]] auto level(GInt cellId) const -> GInt { return m_cellLvl[cellId]; }
+  [[nodiscard]] auto parent(GInt cellId) const -> GInt { return m_parent[cellId]; }
+  // levels that hold cells, ascending (one entry for a single-level grid)
+  [[nodiscard]] auto levels() const -> const std::vector<GInt>& { return m_levels; }
+  // the cells of level lvl as a single-level grid of their own (neighbours,
+  // boundary flags, surfaces and periodic links restricted to the level, a
+  // structured window); ids[q] = this grid's id of its cell q
+  [[nodiscard]] auto levelGrid(GInt lvl, std::vector<GInt>& ids) const -> std::unique_ptr<CartesianGrid>;
   [[nodiscard]] auto boundingBox() const -> const BoundingBox& override { return m_bbox; }
   [[nodiscard]] auto cog() const -> std::vector<GDouble> override { return {m_cog.begin(), m_cog.begin() + m_dim}; }
   [[nodiscard]] auto center(GInt cellId, GInt dir) const -> GDouble override { return m_center[cellId * 3 + dir]; }
@@ -136,7 +156,8 @@ class CartesianGrid : public GridInterface {
   [[nodiscard]] auto surfaces() const -> const std::map<std::string, Surface>& { return m_surfaces; }
   [[nodiscard]] auto neighborStride() const -> GInt { return m_stride; }
 
-  // structured view: window of level indices [lo, lo+n) per axis
+  // structured view: window of level indices [lo, lo+n) per axis (single-level
+  // grids; a multi-level grid has one per level, through levelGrid)
   [[nodiscard]] auto extent(int a) const -> GInt { return m_n[a]; }
   [[nodiscard]] auto structIndex(GInt cellId) const -> GInt { return m_struct[cellId]; }
   [[nodiscard]] auto cellAt(GInt s) const -> GInt { return m_cellOf[s]; }
@@ -149,9 +170,14 @@ class CartesianGrid : public GridInterface {
   void addPeriodicConnection(const Surface& A, const Surface& B);
   void addDiagonalNghbrs();
   void checkPeriodicWrap();
+  void setWindow();
+  struct Refinement;
+  void refineLevel(GInt lvl, bool onlyBndry, GInt maxNoCells, Refinement& st);
 
   int m_dim = 0;
-  GInt m_level = 0;
+  GInt m_level = 0;   // partition level
+  GInt m_maxLvl = 0;  // maxRfnmtLvl
+  GInt m_uniformLvl = 0;
   BoundingBox m_bbox;
   std::array<GDouble, 3> m_cog{{0, 0, 0}};
   std::vector<GDouble> m_len;
@@ -159,6 +185,9 @@ class CartesianGrid : public GridInterface {
   // cells in id order: level indices and centres
   std::vector<std::array<GInt, 3>> m_cell;
   std::vector<GDouble> m_center;
+  std::vector<GInt> m_cellLvl, m_parent;
+  std::vector<uint8_t> m_leaf;
+  std::vector<GInt> m_levels;
   std::vector<GInt> m_nghbr;  // [cell][m_stride]
   GInt m_stride = 0;
   std::vector<uint8_t> m_bndry;

@@ -63,9 +63,21 @@ int lbm_host_grid_bndry(const lbm_host_grid* g, uint8_t* out) {
 }
 
 int lbm_host_grid_structured(const lbm_host_grid* g, int64_t* nxyz, int64_t* index) {
+  if (g->grid->levels().size() != 1) {
+    g_host_err = "lbm_host_grid_structured: a multi-level grid has one structured window per level";
+    return -1;
+  }
   for (int a = 0; a < 3; ++a) nxyz[a] = g->grid->extent(a);
   if (index)
     for (GInt q = 0; q < g->grid->size(); ++q) index[q] = g->grid->structIndex(q);
+  return 0;
+}
+
+int lbm_host_grid_levels(const lbm_host_grid* g, int32_t* level, int64_t* parent) {
+  for (GInt q = 0; q < g->grid->size(); ++q) {
+    if (level) level[q] = (int32_t)g->grid->level(q);
+    if (parent) parent[q] = g->grid->parent(q);
+  }
   return 0;
 }
 

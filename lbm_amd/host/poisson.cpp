@@ -203,6 +203,8 @@ auto PoissonSolver::run() -> GInt {
   std::cerr << m_grid->dim() << "D LBM Solver started ||>" << std::endl;
   loadConfiguration();
   m_pp.init(m_ppConf, *m_grid);
+  if (m_grid->levels().size() != 1)
+    TERMM(-1, "The Poisson solver of this build runs single-level grids (partitionLevel = uniformLevel = maxRfnmtLvl)");
   const GInt nx = m_grid->extent(0), ny = m_grid->extent(1), N = m_grid->size();
   if (nx * ny != N) TERMM(-1, "The Poisson solver needs a full structured window (a box)");
   lbm_poisson_params p{(int32_t)m_lattice, (int32_t)nx, (int32_t)ny, m_omega, m_source};

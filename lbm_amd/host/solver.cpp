@@ -5,6 +5,7 @@
 #include "vtk.h"
 
 #include <cstring>
+#include <set>
 
 namespace lbmh {
 
@@ -54,10 +55,12 @@ auto GridGenerator::run() -> GInt {
 // --------------------------------------------------------------- LBM solver
 LBMSolver::~LBMSolver() {
   if (m_h) lbm_destroy(m_h);
+  for (auto& L : m_lat)
+    if (L.h) lbm_destroy(L.h);
 }
 
-void LBMSolver::check(int rc, const char* what) {
-  if (rc != LBM_OK) TERMM(-1, std::string(what) + ": " + lbm_last_error(m_h));
+void LBMSolver::check(int rc, const char* what, const lbm_handle* h) {
+  if (rc != LBM_OK) TERMM(-1, std::string(what) + ": " + lbm_last_error(h ? h : m_h));
 }
 
 void LBMSolver::init(int, char**, std::string config_file) {
@@ -107,29 +110,39 @@ void LBMSolver::loadConfiguration() {
   if (m_solutionInterval < 1) TERMM(-1, "solution_interval must be >= 1");
   m_solutionFile = c.opt_config_value<std::string>("solution_filename", "solution");
   // output filter (CellFilterManager, src/cell_filter.h:64-103), default
-  // {"cellFilter": "leafCells"} (src/lbm/solver.cpp:86-87); the grids of
-  // this build have one level, so a filter passes every cell or none
+  // {"cellFilter": "leafCells"} (src/lbm/solver.cpp:86-87): leaf cells,
+  // the highest level, the partition level or a given level
   {
     json fc = c.getObject("cellFilter");
     if (fc.empty()) fc = json{{"cellFilter", "leafCells"}};
     if (!fc.count("cellFilter")) TERMM(-1, "Invalid output configuration missing \"cellFilter\" key");
     if (fc["cellFilter"].is_array()) TERMM(-1, "untested");
     const std::string f = fc["cellFilter"].get<std::string>();
-    m_outputAll = true;
+    const GInt N = m_grid->size();
+    m_outputCell.assign(N, 1);
+    GInt lvl = -1;
     if (f == "targetLvl") {
       if (!fc.count("outputLvl")) TERMM(-1, "The required configuration value is missing: outputLvl");
-      const GInt lvl = fc["outputLvl"].get<GInt>();
+      lvl = fc["outputLvl"].get<GInt>();
       if (lvl < m_grid->partitionLvl()) TERMM(-1, "Outputting a lvl below the partition lvl is not possible!");
-      m_outputAll = lvl == m_grid->partitionLvl();
-    } else if (f != "highestLvl" && f != "lowestLvl" && f != "partitionLvl" && f != "leafCells") {
+    } else if (f == "highestLvl") {
+      lvl = m_grid->levels().back();
+    } else if (f == "lowestLvl" || f == "partitionLvl") {
+      lvl = m_grid->partitionLvl();
+    } else if (f != "leafCells") {
       TERMM(-1, "Unknown output filter " + f);
     }
+    for (GInt q = 0; q < N; ++q)
+      m_outputCell[q] = lvl >= 0 ? m_grid->level(q) == lvl : m_grid->property(q, CellProperties::leaf);
   }
   m_ppConf = c.getObject("postprocessing");
   c.opt_config_value<bool>("assumeAxisAligned", true);
   m_refLength = c.opt_config_value<GDouble>("refLength", 1.0);
-  if (c.has_config_value("forcing"))
-    TERMM(-1, "The 'forcing' pressure-gradient hack is not part of this build; use the 'bodyForce' key (Guo forcing)");
+  // LBMSolver::forcing (src/lbm/solver.cpp:626-696): any non-empty value
+  // turns the inlet/outlet correction on, with the pressures 1 +
+  // poiseuillePressureGradient and 1 on the surfaces cube_-x and cube_+x
+  m_forcing = !c.opt_config_value<std::string>("forcing", "").empty();
+  if (m_forcing) m_forcingGrad = c.required_config_value<GDouble>("poiseuillePressureGradient");
   const GDouble finestSpacing = 1.0 / (std::pow((GDouble)m_grid->size(), 1.0 / m_dim) - 1);
   if (c.has_config_value("reynoldsnumber") && c.has_config_value("relaxation"))
     TERMM(-1, "Only set either reynoldsnumber or relaxation");
@@ -174,14 +187,102 @@ void LBMSolver::initialCondition() {
   m_vars.assign(N * nv, 0.0);
   m_bnd.initCndBnd([&](GInt c, GInt v) -> GDouble& { return m_vars[c * nv + v]; });
   for (GInt q = 0; q < N; ++q) m_vars[q * nv + m_dim] = 1.0;
-  const GInt S = m_grid->structSize();
-  std::vector<GDouble> rho(S, 1.0), u(S * m_dim, 0.0);
-  for (GInt q = 0; q < N; ++q) {
-    const GInt s = m_grid->structIndex(q);
-    rho[s] = m_vars[q * nv + m_dim];
-    for (GInt d = 0; d < m_dim; ++d) u[s * m_dim + d] = m_vars[q * nv + d];
+  for (auto& L : m_lat) {
+    const CartesianGrid& G = latGrid(L);
+    const GInt S = G.structSize();
+    std::vector<GDouble> rho(S, 1.0), u(S * m_dim, 0.0);
+    for (GInt q = 0; q < G.size(); ++q) {
+      const GInt s = G.structIndex(q), g = latId(L, q);
+      rho[s] = m_vars[g * nv + m_dim];
+      for (GInt d = 0; d < m_dim; ++d) u[s * m_dim + d] = m_vars[g * nv + d];
+    }
+    check(lbm_init_fields(L.h, rho.data(), u.data()), "lbm_init_fields", L.h);
   }
-  check(lbm_init_fields(m_h, rho.data(), u.data()), "lbm_init_fields");
+}
+
+// one lattice per level (LBMSolver::transferGrid keeps every level of the
+// grid, src/lbm/solver.cpp:759-801, and every step loops over all cells)
+void LBMSolver::createLattices() {
+  const auto& lv = m_grid->levels();
+  if (m_grid->maxLvl() > m_grid->uniformLvl())
+    TERMM(-1, "Boundary refinement (maxRfnmtLvl > uniformLevel) keeps only the cells along the boundary on its levels; "
+              "the reference leaves the populations that would stream in from their missing inner neighbours stale, "
+              "which the device lattices of this build do not reproduce (uniformLevel > partitionLevel is supported)");
+  m_lat.clear();
+  m_lat.resize(lv.size());
+  for (size_t i = 0; i < lv.size(); ++i) {
+    Lattice& L = m_lat[i];
+    if (lv.size() > 1) {
+      L.grid = m_grid->levelGrid(lv[i], L.ids);
+      L.bnd.setupBndryCnds(m_solverCfg->getObject("boundary"), *L.grid);
+    }
+    const CartesianGrid& G = latGrid(L);
+    const DevicePlan plan = (L.grid ? L.bnd : m_bnd).lower(G);
+    lbm_grid_desc g;
+    g.dim = (int32_t)m_dim;
+    g.nx = (int32_t)plan.nx;
+    g.ny = (int32_t)plan.ny;
+    g.nz = (int32_t)plan.nz;
+    g.solid = plan.anySolid ? plan.solid.data() : nullptr;
+    lbm_params p;
+    p.lattice = (int32_t)m_lattice;
+    p.storage = m_storage;
+    p.omega = m_omega;
+    for (int d = 0; d < 3; ++d) p.force[d] = m_force[d];
+    check(lbm_create(&g, &p, &L.h), "lbm_create");
+    check(lbm_set_bc(L.h, plan.faces.data(), (int32_t)plan.faces.size()), "lbm_set_bc", L.h);
+    if (m_forcing) {
+      // the reference takes the surfaces cube_-x and cube_+x and pairs their
+      // cells row by row; on the structured window they must be the x = 0
+      // and x = nx-1 columns
+      if (lv.size() > 1) TERMM(-1, "The 'forcing' correction is not supported on multi-level grids");
+      if (m_dim != 2) TERMM(-1, "The 'forcing' correction is 2D (D2Q9)");
+      if (m_force[0] != 0 || m_force[1] != 0 || m_force[2] != 0) TERMM(-1, "Use either 'forcing' or 'bodyForce'");
+      if (!G.hasBndrySurface("cube_-x") || !G.hasBndrySurface("cube_+x"))
+        TERMM(-1, "The 'forcing' correction needs the surfaces cube_-x and cube_+x");
+      auto column = [&](const char* name, GInt x) {
+        const auto& cells = const_cast<CartesianGrid&>(G).bndrySurface(name).getCellList();
+        std::set<GInt> rows;
+        for (GInt q : cells) {
+          const GInt si = G.structIndex(q);
+          if (si % plan.nx != x) return false;
+          rows.insert(si / plan.nx);
+        }
+        return (GInt)rows.size() == plan.ny;
+      };
+      if (!column("cube_-x", 0) || !column("cube_+x", plan.nx - 1))
+        TERMM(-1, "The 'forcing' correction needs cube_-x and cube_+x to be the full x faces of the grid");
+      check(lbm_set_pressure_forcing(L.h, 1, 1.0 + m_forcingGrad, 1.0), "lbm_set_pressure_forcing", L.h);
+    }
+    if (lv.size() > 1)
+      std::cerr << "Level " << lv[i] << ": " << G.size() << " cells, window " << plan.nx << " x " << plan.ny << " x "
+                << plan.nz << std::endl;
+  }
+}
+
+// one step of every lattice; check steps add their sums (each a sum over
+// the level's cells) into the sums over the whole grid
+auto LBMSolver::stepAll(int mode, lbm_sums* sums) -> int {
+  int out = LBM_OK;
+  if (sums) *sums = lbm_sums{};
+  lbm_sums s{};
+  for (auto& L : m_lat) {
+    s = lbm_sums{};
+    const int rc = lbm_step_ex(L.h, mode, sums ? &s : nullptr);
+    if (rc == LBM_DIVERGED) out = LBM_DIVERGED;
+    else check(rc, "lbm_step_ex", L.h);
+    if (!sums) continue;
+    sums->mass += s.mass;
+    for (int d = 0; d < 3; ++d) sums->momentum[d] += s.momentum[d];
+    for (int v = 0; v < 4; ++v) sums->abs_diff[v] += s.abs_diff[v];
+    sums->diff_sq += s.diff_sq;
+    sums->norm_sq += s.norm_sq;
+    sums->nonfinite += s.nonfinite;
+    sums->fluid_cells += s.fluid_cells;
+  }
+  // one lattice: the residual exactly as its step reported it
+  if (sums) sums->res_rel = m_lat.size() == 1 ? s.res_rel : (sums->norm_sq > 0 ? std::sqrt(sums->diff_sq / sums->norm_sq) : 0.0);
+  return out;
 }
 
 auto LBMSolver::convergenceCondition(const lbm_sums& s) -> bool {
@@ -210,16 +311,19 @@ auto LBMSolver::convergenceCondition(const lbm_sums& s) -> bool {
 }
 
 void LBMSolver::fetchMacros() {
-  const GInt S = m_grid->structSize();
-  std::vector<GDouble> rho(S), u(S * m_dim);
-  check(lbm_get_macros(m_h, rho.data(), u.data()), "lbm_get_macros");
   const GInt N = m_grid->size();
   m_rho.assign(N, 0.0);
   m_u.assign(N * m_dim, 0.0);
-  for (GInt q = 0; q < N; ++q) {
-    const GInt s = m_grid->structIndex(q);
-    m_rho[q] = rho[s];
-    for (GInt d = 0; d < m_dim; ++d) m_u[q * m_dim + d] = u[s * m_dim + d];
+  for (auto& L : m_lat) {
+    const CartesianGrid& G = latGrid(L);
+    const GInt S = G.structSize();
+    std::vector<GDouble> rho(S), u(S * m_dim);
+    check(lbm_get_macros(L.h, rho.data(), u.data()), "lbm_get_macros", L.h);
+    for (GInt q = 0; q < G.size(); ++q) {
+      const GInt s = G.structIndex(q), g = latId(L, q);
+      m_rho[g] = rho[s];
+      for (GInt d = 0; d < m_dim; ++d) m_u[g * m_dim + d] = u[s * m_dim + d];
+    }
   }
 }
 
@@ -229,20 +333,7 @@ auto LBMSolver::run() -> GInt {
   std::cerr << m_grid->dim() << "D LBM Solver started ||>" << std::endl;
   loadConfiguration();
   m_pp.init(m_ppConf, *m_grid);  // Postprocess::init (src/lbm/solver.cpp:180)
-  const DevicePlan plan = m_bnd.lower(*m_grid);
-  lbm_grid_desc g;
-  g.dim = (int32_t)m_dim;
-  g.nx = (int32_t)plan.nx;
-  g.ny = (int32_t)plan.ny;
-  g.nz = (int32_t)plan.nz;
-  g.solid = plan.anySolid ? plan.solid.data() : nullptr;
-  lbm_params p;
-  p.lattice = (int32_t)m_lattice;
-  p.storage = m_storage;
-  p.omega = m_omega;
-  for (int d = 0; d < 3; ++d) p.force[d] = m_force[d];
-  check(lbm_create(&g, &p, &m_h), "lbm_create");
-  check(lbm_set_bc(m_h, plan.faces.data(), (int32_t)plan.faces.size()), "lbm_set_bc");
+  createLattices();
   initialCondition();
   // postprocessing (src/lbm/solver.cpp:187,199,309,318): the functions read
   // the u held by the solver, which the reference refreshes at the start of
@@ -284,22 +375,16 @@ auto LBMSolver::run() -> GInt {
       for (GInt q = 0; q < m_grid->size(); ++q) m_ppU[q] = m_u[q * m_dim];
     }
     lbm_sums s{};
-    const int rc = lbm_step_ex(m_h, mode, chk ? &s : nullptr);
-    if (rc == LBM_DIVERGED) {
+    const int rc = stepAll(mode, chk ? &s : nullptr);
+    if (rc == LBM_DIVERGED || chk) {
       last = s;
       haveSums = true;
-    } else {
-      check(rc, "lbm_step_ex");
-      if (chk) {
-        last = s;
-        haveSums = true;
-      }
     }
     // also at the last step or once converged (src/lbm/solver.cpp:194-195)
     m_pp.execute(PPHook::AfterTimestep, *m_grid, m_ppU);
     output(m_timeStep == m_maxSteps - 1 || converged, "");
   }
-  check(lbm_synchronize(m_h), "lbm_synchronize");
+  for (auto& L : m_lat) check(lbm_synchronize(L.h), "lbm_synchronize", L.h);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::cerr << "Main loop: " << m_timeStep << " steps in " << secs << " s ("
             << (double)m_timeStep * m_grid->size() / std::max(secs, 1e-12) / 1e6 << " MLUPS)" << std::endl;
@@ -385,7 +470,12 @@ void LBMSolver::output(bool forced, const std::string& postfix) {
   fetchMacros();
   const GInt N = m_grid->size();
   for (GInt q = 0; q < N; ++q) m_ppU[q] = m_u[q * m_dim];
-  const GInt n = m_outputAll ? N : 0;
+  // the cells the filter passes, in cell order (VTK::BINARY::writePoints,
+  // src/common/IO.h:414-507)
+  std::vector<GInt> cells;
+  for (GInt q = 0; q < N; ++q)
+    if (m_outputCell[q]) cells.push_back(q);
+  const GInt n = (GInt)cells.size();
   if (n == 0) {
     std::cerr << "ERROR: Invalid call to encodeLE() with length = 0" << std::endl;
     std::exit(-1);
@@ -393,19 +483,19 @@ void LBMSolver::output(bool forced, const std::string& postfix) {
   static const char* velName[3] = {"U", "V", "W"};  // VAR::VELSTR, src/lbm/variables.h:70
   std::vector<vtk::Field> fields;
   for (GInt d = 0; d < m_dim; ++d) {
-    vtk::Field f{velName[d], std::vector<double>((size_t)N)};
-    for (GInt q = 0; q < N; ++q) f.values[q] = vtk::textRoundTrip(m_u[q * m_dim + d]);
+    vtk::Field f{velName[d], std::vector<double>((size_t)n)};
+    for (GInt k = 0; k < n; ++k) f.values[k] = vtk::textRoundTrip(m_u[cells[k] * m_dim + d]);
     fields.push_back(std::move(f));
   }
-  vtk::Field r{"rho", std::vector<double>((size_t)N)};
-  for (GInt q = 0; q < N; ++q) r.values[q] = vtk::textRoundTrip(m_rho[q]);
+  vtk::Field r{"rho", std::vector<double>((size_t)n)};
+  for (GInt k = 0; k < n; ++k) r.values[k] = vtk::textRoundTrip(m_rho[cells[k]]);
   fields.push_back(std::move(r));
-  std::vector<double> xyz((size_t)N * m_dim);
-  for (GInt q = 0; q < N; ++q)
-    for (GInt d = 0; d < m_dim; ++d) xyz[q * m_dim + d] = m_grid->center(q, d);
+  std::vector<double> xyz((size_t)n * m_dim);
+  for (GInt k = 0; k < n; ++k)
+    for (GInt d = 0; d < m_dim; ++d) xyz[k * m_dim + d] = m_grid->center(cells[k], d);
   mkdirs(m_outputDir);
   const std::string base = m_outputDir + m_solutionFile + "_" + std::to_string(m_timeStep) + postfix;
-  if (!vtk::writePoints(base, (int)m_dim, N, xyz.data(), fields))
+  if (!vtk::writePoints(base, (int)m_dim, n, xyz.data(), fields))
     std::cerr << "WARNING: cannot write " << base << ".vtp" << std::endl;
 }
 

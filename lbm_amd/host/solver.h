@@ -63,20 +63,36 @@ class LBMSolver : public Runnable {
 
  private:
   void loadConfiguration();
+  void createLattices();
   void initialCondition();
   auto convergenceCondition(const lbm_sums& s) -> bool;
   void fetchMacros();
+  auto stepAll(int mode, lbm_sums* sums) -> int;
   void compareToAnalyticalResult();
   void output(bool forced, const std::string& postfix);
   auto runBenchmark() -> GInt;
-  void check(int rc, const char* what);
+  void check(int rc, const char* what, const lbm_handle* h = nullptr);
 
   SolverOptions m_opt;
   json m_config;
   std::unique_ptr<Configuration> m_solverCfg;
   std::unique_ptr<CartesianGrid> m_grid;
   LBMBndManager m_bnd;
-  lbm_handle* m_h = nullptr;
+  lbm_handle* m_h = nullptr;  // `lbm --bench`
+  // One device lattice per grid level (a single-level grid has one; the
+  // reference steps every level of a multi-level grid as its own lattice,
+  // its cells linked only within the level): the level's grid and boundary
+  // objects (multi-level grids; a single-level lattice uses m_grid and
+  // m_bnd), ids[q] = m_grid's id of the lattice's cell q (empty: the same)
+  struct Lattice {
+    std::unique_ptr<CartesianGrid> grid;
+    std::vector<GInt> ids;
+    LBMBndManager bnd;
+    lbm_handle* h = nullptr;
+  };
+  std::vector<Lattice> m_lat;
+  [[nodiscard]] auto latGrid(const Lattice& L) const -> const CartesianGrid& { return L.grid ? *L.grid : *m_grid; }
+  [[nodiscard]] static auto latId(const Lattice& L, GInt q) -> GInt { return L.ids.empty() ? q : L.ids[q]; }
   bool m_benchmark = false;
 
   // configuration (src/lbm/solver.cpp:71-163)
@@ -87,10 +103,12 @@ class LBMSolver : public Runnable {
   GDouble m_refLength = 1.0, m_ma = 0.01, m_relaxTime = 0.9, m_omega = 1.0 / 0.9, m_nu = 0, m_re = 1, m_dt = NAN;
   GDouble m_convergence = 1e-12;
   std::array<GDouble, 3> m_force{{0, 0, 0}};
+  bool m_forcing = false;  // the reference's "forcing" inlet/outlet correction
+  GDouble m_forcingGrad = 0;
   int m_storage = LBM_STORE_F32;
   GInt m_timeStep = 0;
   bool m_diverged = false, m_wroteDiverged = false;
-  bool m_outputAll = true;  // the output filter passes every cell (else none)
+  std::vector<uint8_t> m_outputCell;  // the output filter's verdict per cell
   std::vector<GDouble> m_vars;  // host initial values [cell][dim+1]
   std::vector<GDouble> m_rho, m_u;
   json m_ppConf;

@@ -9,8 +9,17 @@ swap-with-last compaction, reorderHilbertCurve's argsort + swap procedure --
 so that the product host (lbm_amd/host/grid.cpp, which works on a structured
 window instead) can be checked against it cell id by cell id.
 
+Multi-level grids: uniformRefineGrid (every cell of a level refined, children
+appended after the cells so far) and markBndryCells / refineMarkedCells
+(the boundary cells of the finest level refined), each followed by the same
+neighbour search and deletion; the solver side tests boundary cells by the
+parent rule at each level's length.  Pinned by tests/test_grid.py
+(test_multilevel_restatement_invariants: parent/child geometry, closed-form
+level sizes and child ids on the unit square, brute-force neighbour links,
+boundary refinement of exactly the boundary cells).
+
 Reference: src/gridgenerator/cartesiangrid_generation.h:71-147 (partition
-grid), :343-380 (refineGrid), :419-451 (refineCell), :453-506 (neighbours),
+grid), :151-206 (uniform and boundary refinement), :343-380 (refineGrid), :419-451 (refineCell), :453-506 (neighbours),
 :509-620 (deletion, flood fill), :637-661 (copyCell/swapCell/updateParent),
 :664-735 (Hilbert reorder); src/cartesiangrid.h:451-493 (diagonals),
 :510-546 (boundary cells), :553-603 (surfaces), :608-706 (periodic links);
@@ -158,7 +167,9 @@ class Generator:
     def __init__(self, cfg):
         self.dim = cfg["dim"]
         self.L = cfg["partitionLevel"]
-        assert cfg["uniformLevel"] == self.L and cfg.get("maxRfnmtLvl", self.L) == self.L
+        self.U = cfg["uniformLevel"]
+        self.M = cfg.get("maxRfnmtLvl", self.U)
+        assert self.L <= self.U <= self.M
         self.geom = Geometry(cfg["geometry"], self.dim)
         self.align = bool(cfg.get("alignNodesWithSurface", False))
         self.align_dir = int(cfg.get("alignDir", 1))
@@ -169,11 +180,11 @@ class Generator:
             decisive = d if ext[d] > ext[decisive] else decisive
         self.cog = [mn[d] + 0.5 * (mx[d] - mn[d]) for d in range(self.dim)]
         self.len = [(1.0 + 1.0 / math.pow(2.0, MAX_LVL)) * ext[decisive]]
-        for _ in range(self.L):
+        for _ in range(self.M):
             self.len.append(0.5 * self.len[-1])
         self.nch = 1 << self.dim
         self.nn = 2 * self.dim
-        n_est = sum((1 << (self.dim * l)) for l in range(self.L + 1))
+        n_est = sum((1 << (self.dim * l)) for l in range(self.M + 1))
         self.cap = min(cfg["maxNoCells"], 4 * n_est + 16)
         cap = self.cap
         self.center = [None] * cap
@@ -185,7 +196,7 @@ class Generator:
         self.bndry = [False] * cap
         self.inside = [False] * cap
         self.marked = [False] * cap
-        self.offsets = [None] * (self.L + 1)
+        self.offsets = [None] * (self.M + 1)
 
     # createPartitioningGrid
     def run(self):
@@ -215,9 +226,38 @@ class Generator:
         self.size = self.offsets[self.L][1] - self.offsets[self.L][0]
         assert self.offsets[self.L][0] == 0
         self.reorder_hilbert()
+        # uniformRefineGrid (cartesiangrid_generation.h:151-169): every cell of
+        # a level gets its children, appended after the cells so far
+        highest = self.L
+        for lvl in range(self.L, self.U):
+            b, e = self.offsets[lvl]
+            self.offsets[lvl + 1] = [self.size, self.size + (e - b) * nch]
+            for cid in range(b, e):
+                self.refine_cell(cid, self.offsets[lvl + 1][0] + (cid - b) * nch)
+            self.refine_level_done(lvl)
+            highest += 1
+        # boundary refinement (gridGenerator.cpp:209-213, markBndryCells /
+        # refineMarkedCells, cartesiangrid_generation.h:171-206,409-417): the
+        # boundary cells of the finest level get children
+        for _ in range(self.U, self.M):
+            b, e = self.offsets[highest]
+            marked = [c for c in range(b, e) if self.bndry[c]]
+            self.offsets[highest + 1] = [self.size, self.size + len(marked) * nch]
+            for k, cid in enumerate(marked):
+                self.refine_cell(cid, self.offsets[highest + 1][0] + k * nch)
+            self.refine_level_done(highest)
+            highest += 1
         if self.align:
             self.align_with_surface(self.align_dir)
         return self
+
+    def refine_level_done(self, lvl):
+        """refineGrid's tail (cartesiangrid_generation.h:362-378): neighbours
+        of the new level, deletion of its outside cells."""
+        self.size = self.offsets[lvl + 1][1]
+        self.find_child_nghbrs(lvl)
+        self.delete_outside(lvl + 1)
+        self.size = self.offsets[lvl + 1][1]
 
     def align_with_surface(self, align_dir):
         """transformMaxRfnmtLvlToExtent
@@ -226,7 +266,7 @@ class Generator:
         box along align_dir (along every axis when the nodal extent is the
         same in all of them), scale the cell length alike, then delete the
         cells that are neither cut nor inside."""
-        b, e = self.offsets[self.L]
+        b, e = self.offsets[self.M]
         mn = [sys.float_info.max] * self.dim
         mx = [sys.float_info.min] * self.dim
         for c in range(b, e):
@@ -250,15 +290,15 @@ class Generator:
                     self.center[c][d] = self.center[c][d] * T - (T * mx[d] - bmax[d])
                 else:
                     self.center[c][d] = self.center[c][d] * T
-        self.len[self.L] *= T
+        self.len[self.M] *= T
         for c in range(b, e):
-            isb = self.geom.cut(self.center[c], self.len[self.L])
+            isb = self.geom.cut(self.center[c], self.len[self.M])
             self.bndry[c] = isb
             self.inside[c] = isb or self.geom.inside(self.center[c])
         for c in range(e - 1, b - 1, -1):
             if not self.inside[c]:
                 self.delete_cell(c)
-        self.size = self.offsets[self.L][1] - self.offsets[self.L][0]
+        self.size = self.offsets[self.M][1]
 
     def refine_cell(self, cid, off):
         lvl = self.level[cid] + 1
@@ -407,14 +447,24 @@ class SolverGrid:
         self.center = [list(gen.center[c]) for c in range(n)]
         nn = 2 * self.dim
         main = [list(gen.nghbr[c]) for c in range(n)]
-        L = gen.L
-        ln = gen.len[L]
-        # determineBoundaryCells
+        self.level = [gen.level[c] for c in range(n)]
+        self.parent = [gen.parent[c] for c in range(n)]
+
+        def ln(c):
+            return gen.len[self.level[c]]
+
+        # determineBoundaryCells (src/cartesiangrid.h:510-546): a cell of the
+        # partition level, or a child of a boundary cell, is one if the
+        # geometry cuts it at its level's length and a main neighbour is
+        # missing (parents come before their children in cell order)
         self.bndry = []
         for c in range(n):
-            b = gen.geom.cut(self.center[c], ln)
-            if b and all(main[c][d] != -1 for d in range(nn)):
-                b = False
+            b = False
+            p = self.parent[c]
+            if p == -1 or self.bndry[p]:
+                b = gen.geom.cut(self.center[c], ln(c))
+                if b and all(main[c][d] != -1 for d in range(nn)):
+                    b = False
             self.bndry.append(b)
         # identifyBndrySurfaces
         bcfg = solver_cfg.get("boundary", {})
@@ -432,7 +482,7 @@ class SolverGrid:
                     for c in range(n):
                         if not self.bndry[c] or main[c][d] != -1:
                             continue
-                        if not gen.geom.cut(self.center[c], ln, gname):
+                        if not gen.geom.cut(self.center[c], ln(c), gname):
                             continue
                         if (c, d) in assigned:
                             continue

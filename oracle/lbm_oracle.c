@@ -261,8 +261,30 @@ static void zou_he_cell(const lattice_t *L, double *f, int axis, int sign, int i
  * vars receives macros(fold) at the start of the step (what the reference
  * holds in m_vars after this step).
  */
+static int step_impl(int Q, int64_t ncell, const int64_t *nbr, double omega, const double *force, double *fold,
+                     double *fpost, double *vars, int64_t nbc, const orc_bc *bcs, int64_t nforce, const int64_t *fcell,
+                     const int64_t *vcell, const double *fpres);
+
 int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const double *force, double *fold,
              double *fpost, double *vars, int64_t nbc, const orc_bc *bcs) {
+  return step_impl(Q, ncell, nbr, omega, force, fold, fpost, vars, nbc, bcs, 0, NULL, NULL, NULL);
+}
+
+/* The same step with LBMSolver::forcing (src/lbm/solver.cpp:626-696)
+ * between the collision and the propagation: cell fcell[k] takes
+ * f_i = feq_i(fpres[k], u_v) + f_i(v) - feq_i(rho_v, u_v) from its value
+ * cell v = vcell[k] (post-collision f, the step's macros and equilibrium of
+ * v), where feq_i(p, u_v) puts only u_v's x component into c_i.u and all
+ * of u_v into u.u (the reference's cu = velocity(v, 0) * dirs[i][0]). */
+int orc_step_forced(int Q, int64_t ncell, const int64_t *nbr, double omega, const double *force, double *fold,
+                    double *fpost, double *vars, int64_t nbc, const orc_bc *bcs, int64_t nforce, const int64_t *fcell,
+                    const int64_t *vcell, const double *fpres) {
+  return step_impl(Q, ncell, nbr, omega, force, fold, fpost, vars, nbc, bcs, nforce, fcell, vcell, fpres);
+}
+
+static int step_impl(int Q, int64_t ncell, const int64_t *nbr, double omega, const double *force, double *fold,
+                     double *fpost, double *vars, int64_t nbc, const orc_bc *bcs, int64_t nforce, const int64_t *fcell,
+                     const int64_t *vcell, const double *fpres) {
   lattice_t L;
   if (get_lattice(Q, &L)) return -1;
   const int nv = L.dim + 1;
@@ -287,6 +309,20 @@ int orc_step(int Q, int64_t ncell, const int64_t *nbr, double omega, const doubl
         for (int d = 0; d < L.dim; ++d) s += ((L.c[i][d] - v[d]) / CSSQ + cu * L.c[i][d] / (CSSQ * CSSQ)) * force[d];
         fp[i] += (1 - 0.5 * omega) * L.w[i] * s;
       }
+    }
+  }
+
+  /* forcing: the face cells from their value cells, in list order */
+  for (int64_t k = 0; k < nforce; ++k) {
+    const int64_t c = fcell[k], v = vcell[k];
+    const double *vv = vars + v * nv;
+    double vsq = 0;
+    for (int d = 0; d < L.dim; ++d) vsq += vv[d] * vv[d];
+    double feqv[19];
+    feq_all(&L, vv[L.dim], vv, feqv);
+    for (int i = 0; i < Q; ++i) {
+      const double cu = vv[0] * L.c[i][0];
+      fpost[c * Q + i] = default_eq(L.w[i], fpres[k], cu, vsq) + fpost[v * Q + i] - feqv[i];
     }
   }
 

@@ -84,6 +84,8 @@ def lib():
         L.orc_macros.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
         L.orc_step.argtypes = [ct.c_int, ct.c_int64, i64p, ct.c_double, dp, dp, dp, dp, ct.c_int64,
                                ct.POINTER(OrcBC)]
+        L.orc_step_forced.argtypes = [ct.c_int, ct.c_int64, i64p, ct.c_double, dp, dp, dp, dp, ct.c_int64,
+                                      ct.POINTER(OrcBC), ct.c_int64, i64p, i64p, dp]
         L.orc_init.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
         L.orc_residual.argtypes = [ct.c_int, ct.c_int64, dp, dp, dp]
         L.orc_hilbert_index.argtypes = [ct.c_int, dp, ct.c_int64]
@@ -600,6 +602,27 @@ class GridProblem(Problem):
         allr = pre + recs + zh
         self.nrec = len(allr)
         self.recs = (OrcBC * max(1, len(allr)))(*allr)
+
+    def set_forcing(self, cells, value_cells, pressures):
+        """LBMSolver::forcing (src/lbm/solver.cpp:626-696): after every
+        collision cell cells[k] takes feq(pressures[k], u_v) + f(v) - feq(v)
+        from v = value_cells[k] (orc_step_forced)."""
+        self.forcing = (np.ascontiguousarray(cells, dtype=np.int64), np.ascontiguousarray(value_cells, dtype=np.int64),
+                        np.ascontiguousarray(pressures, dtype=np.float64))
+
+    def step(self, n=1):
+        fz = getattr(self, "forcing", None)
+        if fz is None or len(fz[0]) == 0:
+            return super().step(n)
+        N = int(np.prod(self.shape))
+        L = lib()
+        i64 = ct.POINTER(ct.c_int64)
+        for _ in range(int(n)):
+            rc = L.orc_step_forced(self.Q, N, self.nbr.ctypes.data_as(i64), self.omega, _dp(self.force), _dp(self.fold),
+                                   _dp(self.fpost), _dp(self.vars), self.nrec, self.recs, len(fz[0]),
+                                   fz[0].ctypes.data_as(i64), fz[1].ctypes.data_as(i64), _dp(fz[2]))
+            assert rc == 0, rc
+            self.steps += 1
 
     def init_reference(self):
         """initialCondition (src/lbm/solver.cpp:267-295): rho = 1, u from the

@@ -245,6 +245,30 @@ def oracle_run(cfg, h, steps, textbook=False):
 
 
 def wrapped_neighbours(h, cfg):
+    """Per level on a multi-level grid (each level is its own lattice)."""
+    lv = h.get("level")
+    if lv is None or len(set(lv.tolist())) == 1:
+        return wrapped_neighbours_window(h, cfg)
+    nb = np.full_like(h["nb"], -1)
+    for l in sorted(set(lv.tolist())):
+        ids = np.nonzero(lv == l)[0]
+        loc = -np.ones(h["n"], dtype=np.int64)
+        loc[ids] = np.arange(ids.size)
+        cen = h["cen"][ids]
+        nxyz = np.ones(3, dtype=np.int64)
+        idx = np.zeros((ids.size, 3), dtype=np.int64)
+        for a in range(h["dim"]):
+            vals, inv = np.unique(cen[:, a], return_inverse=True)
+            nxyz[a] = vals.size
+            idx[:, a] = inv
+        sub = dict(n=ids.size, dim=h["dim"], nb=h["nb"][ids], nxyz=nxyz,
+                   sidx=idx[:, 0] + nxyz[0] * (idx[:, 1] + nxyz[1] * idx[:, 2]))
+        w = wrapped_neighbours_window(sub, cfg)
+        nb[ids] = np.where(w >= 0, ids[np.maximum(w, 0)], -1)
+    return nb
+
+
+def wrapped_neighbours_window(h, cfg):
     """'periodic' boundary objects (generateBndry = true) copy outgoing
     populations to the linked cell of the other face (bnd_periodic.h:102-119),
     i.e. the structured window wraps along that axis."""
@@ -355,3 +379,55 @@ def test_host_matches_oracle_on_reference_grid(name, folder, steps, textbook, ju
     er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
     print(f"{name} {storage}: rel-L2 u={eu:.3e} rho={er:.3e}")
     assert eu < tol and er < tol
+
+
+MULTI_CASES = [
+    ("pressure_channel2d.json", OWNCFG, (3, 5, 5), 300, False),
+    ("pressure_channel2d.json", OWNCFG, (4, 5, 5), 400, False),
+    ("neem_channel2d.json", OWNCFG, (4, 5, 5), 400, False),
+    ("couette__couette_bnd_NEEM.json", REFCFG, (4, 5, 5), 300, False),
+    ("sphere3d_neem.json", OWNCFG, (4, 5, 5), 100, True),
+]
+
+
+@pytest.mark.parametrize("name,folder,levels,steps,textbook", MULTI_CASES)
+def test_host_multilevel_matches_oracle(name, folder, levels, steps, textbook, tmp_path):
+    """partitionLevel < uniformLevel: the reference keeps every level in the
+    grid and its solver loops over all cells, so each level is a lattice of
+    its own (links never cross levels).  The host runs one device lattice
+    per level in lockstep (convergence sums added over the levels); its
+    fields after maxSteps iterations equal the oracle stepped on the
+    restated multi-level grid (one neighbour table over all cells) in FP64."""
+    cfg = json.load(open(os.path.join(folder, name)))
+    cfg["partitionLevel"], cfg["uniformLevel"], cfg["maxRfnmtLvl"] = levels
+    cfg["maxNoCells"] = max(cfg["maxNoCells"], 4_000_000)
+    s = cfg["solver"]
+    s["maxSteps"] = steps
+    s["convergence"] = 0.0
+    for k in ("analyticalSolution", "errorL2", "errorMax", "errorGRE", "analyticalSolutionExcludeSurface",
+              "postprocessing", "poiseuillePressureGradient"):
+        s.pop(k, None)
+    print(f"multi-level {name} levels {levels}", flush=True)
+    h, rho, u, done = host_run(cfg, tmp_path, "f64")
+    assert done == steps
+    from tests.test_grid import oracle_grid
+    g = oracle_grid(cfg)
+    assert h["n"] == g["n"] and len(set(g["level"].tolist())) == levels[1] - levels[0] + 1
+    assert np.array_equal(h["cen"], g["cen"]) and np.array_equal(h["nb"], g["nb"])
+    assert np.array_equal(h["bnd"].astype(bool), g["bnd"].astype(bool)) and h["surf"] == g["surf"]
+    r_rho, r_u = oracle_run(cfg, g, steps, textbook)
+    eu = np.linalg.norm(u - r_u) / np.linalg.norm(r_u)
+    er = np.linalg.norm(rho - r_rho) / np.linalg.norm(r_rho)
+    print(f"{name} levels {levels}: rel-L2 u={eu:.3e} rho={er:.3e}")
+    assert eu < 1e-10 and er < 1e-10
+
+
+def test_cli_refuses_boundary_refinement(tmp_path):
+    """maxRfnmtLvl > uniformLevel: grids are generated (and checked against
+    the restatement on the CPU), the solver ends with status 255."""
+    cfg = json.load(open(os.path.join(REFCFG, "couette__couette_bnd_NEEM.json")))
+    cfg["partitionLevel"], cfg["uniformLevel"], cfg["maxRfnmtLvl"] = 4, 4, 5
+    cfg["solver"]["maxSteps"] = 10
+    r = run_cli(cfg, tmp_path, "-p", "f64", timeout=120)
+    assert r.returncode == 255
+    assert "Boundary refinement" in r.stderr
