@@ -60,9 +60,10 @@ struct ConvArgs {
   // MODE 0: y = bf16(act(acc + bias)), [p][co]
   u16 *y;
   int relu;
-  // MODE 1: channels < c0 accumulate into g0 [p][c0], the rest into g1 [p][co - c0]
+  // MODE 1: channels < c0 are stored into g0 [p][c0], the rest into g1
+  // [p][co - c0], added to what g1 holds when add1 is set
   float *g0, *g1;
-  int c0;
+  int c0, add1;
   // MODE 2: yf = acc + bias for the first cf channels, [p][cf]
   float *yf;
   int cf;
@@ -152,8 +153,12 @@ __global__ void __launch_bounds__(256) k_conv(const ConvArgs a, const int ksplit
       if (a.relu) v = fmaxf(v, 0.f);
       a.y[pp * a.co + co_] = f2bf(v);
     } else if (MODE == 1) {
-      if (co_ < a.c0) a.g0[pp * a.c0 + co_] += v;
-      else a.g1[pp * (a.co - a.c0) + (co_ - a.c0)] += v;
+      if (co_ < a.c0) {
+        a.g0[pp * a.c0 + co_] = v;
+      } else {
+        float *d = a.g1 + pp * (a.co - a.c0) + (co_ - a.c0);
+        *d = a.add1 ? *d + v : v;
+      }
     } else {
       if (co_ < a.cf) a.yf[pp * a.cf + co_] = v + a.bias[co_];
     }
@@ -232,8 +237,12 @@ __global__ void __launch_bounds__(256) k_conv_lds(const ConvArgs a) {
       if (a.relu) v = fmaxf(v, 0.f);
       a.y[pp * a.co + co_] = f2bf(v);
     } else if (MODE == 1) {
-      if (co_ < a.c0) a.g0[pp * a.c0 + co_] += v;
-      else a.g1[pp * (a.co - a.c0) + (co_ - a.c0)] += v;
+      if (co_ < a.c0) {
+        a.g0[pp * a.c0 + co_] = v;
+      } else {
+        float *d = a.g1 + pp * (a.co - a.c0) + (co_ - a.c0);
+        *d = a.add1 ? *d + v : v;
+      }
     } else {
       if (co_ < a.cf) a.yf[pp * a.cf + co_] = v + a.bias[co_];
     }
@@ -566,9 +575,8 @@ __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
 
 // dst[c] = sum over rows r of part[r][c] (n columns): block c, thread t sums
 // rows t, t + 256, ... in order, then a fixed tree
-__global__ void __launch_bounds__(256) k_reduce_rows(const float *part, long long nrows, int n, float *dst) {
+__device__ void reduce_rows(const float *part, long long nrows, int n, float *dst, int c) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
   float s = 0.f;
   for (long long r = threadIdx.x; r < nrows; r += 256) s += part[r * n + c];
   red[threadIdx.x] = s;
@@ -583,10 +591,10 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const float *part, long lon
 // dst[i] = sum over chunks of part[chunk][i]: a block takes 32 elements,
 // its eight 32-lane groups each sum one eighth of the chunks in order, and
 // the eight partial sums are added in group order
-__global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long long n, int nchunks, float *dst) {
+__device__ void reduce_chunks(const float *part, long long n, int nchunks, float *dst, long long blk) {
   __shared__ float red[8][32];
   const int e = threadIdx.x & 31, q = threadIdx.x >> 5;
-  const long long i = blockIdx.x * 32LL + e;
+  const long long i = blk * 32LL + e;
   float s = 0.f;
   if (i < n) {
     const int c1 = (int)((long long)(q + 1) * nchunks / 8);
@@ -608,6 +616,20 @@ __global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long l
     for (int k = 0; k < 8; ++k) t += red[k][e];
     dst[i] = t;
   }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_chunks(const float *part, long long n, int nchunks, float *dst) {
+  reduce_chunks(part, n, nchunks, dst, blockIdx.x);
+}
+
+// one launch for a layer's two parameter gradients: blocks below nbw sum the
+// weight-gradient chunks (32 weights a block), the co blocks after them the
+// bias partial rows
+__global__ void __launch_bounds__(256) k_reduce_wb(const float *part, long long nw, int nchunks, float *dw,
+                                                   const float *bpart, long long nrows, int co, float *db) {
+  const long long nbw = (nw + 31) / 32;
+  if ((long long)blockIdx.x < nbw) reduce_chunks(part, nw, nchunks, dw, blockIdx.x);
+  else reduce_rows(bpart, nrows, co, db, (int)(blockIdx.x - nbw));
 }
 
 // bias gradient partials: part[chunk][c] = sum over the chunk's pixels of
@@ -662,20 +684,22 @@ __global__ void __launch_bounds__(256) k_pool(const u16 *x, u16 *y, int nb, int 
   y[i] = f2bf(0.25f * s);
 }
 
-// gx (full resolution, size 2h x 2w) += gy / 4 at the parent cell
-__global__ void __launch_bounds__(256) k_pool_bwd(const float *gy, float *gx, int nb, int h, int w, int c) {
+// gx (full resolution, size 2h x 2w) = gy / 4 at the parent cell, added to
+// what gx holds when add is set
+__global__ void __launch_bounds__(256) k_pool_bwd(const float *gy, float *gx, int nb, int h, int w, int c, int add) {
   const long long i = blockIdx.x * 256LL + threadIdx.x;
   const long long n = (long long)nb * 2 * h * 2 * w * c;
   if (i >= n) return;
   const int ch = (int)(i % c);
   long long p = i / c;
   const int xx = (int)(p % (2 * w)), yy = (int)((p / (2 * w)) % (2 * h)), b = (int)(p / (4LL * w * h));
-  gx[i] += 0.25f * gy[(((long long)b * h + (yy >> 1)) * w + (xx >> 1)) * c + ch];
+  const float v = 0.25f * gy[(((long long)b * h + (yy >> 1)) * w + (xx >> 1)) * c + ch];
+  gx[i] = add ? gx[i] + v : v;
 }
 
-// ga (half resolution h x w) += sum of the 2x2 children of gup (nearest
-// upsampling backward)
-__global__ void __launch_bounds__(256) k_up_bwd(const float *gup, float *ga, int nb, int h, int w, int c) {
+// ga (half resolution h x w) = sum of the 2x2 children of gup (nearest
+// upsampling backward), added to what ga holds when add is set
+__global__ void __launch_bounds__(256) k_up_bwd(const float *gup, float *ga, int nb, int h, int w, int c, int add) {
   const long long i = blockIdx.x * 256LL + threadIdx.x;
   const long long n = (long long)nb * h * w * c;
   if (i >= n) return;
@@ -683,7 +707,8 @@ __global__ void __launch_bounds__(256) k_up_bwd(const float *gup, float *ga, int
   long long p = i / c;
   const int xx = (int)(p % w), yy = (int)((p / w) % h), b = (int)(p / ((long long)w * h));
   const long long base = (((long long)b * 2 * h + 2 * yy) * 2 * w + 2 * xx) * c + ch;
-  ga[i] += gup[base] + gup[base + c] + gup[base + 2LL * w * c] + gup[base + 2LL * w * c + c];
+  const float v = gup[base] + gup[base + c] + gup[base + 2LL * w * c] + gup[base + 2LL * w * c + c];
+  ga[i] = add ? ga[i] + v : v;
 }
 
 // features [p][cin] FP32 -> bf16 [p][16] (zero padded)
@@ -832,6 +857,9 @@ struct Layer {
   long long woff = 0, boff = 0;  // offsets into the flat parameter array
   u16 *wb = nullptr, *wd = nullptr;
   bool last = false;
+  // the input gradient of this layer is the first write into the s (a)
+  // tensor's gradient in backward order: stored, not added
+  bool s_first = false, a_first = false;
 };
 
 }  // namespace
@@ -842,6 +870,7 @@ struct lbm_sur {
   std::vector<Layer> L;
   std::vector<int> pools;  // tensor pairs (src, dst) of the pooling steps, in forward order
   std::vector<std::pair<int, int>> pool_pairs;
+  std::vector<char> pool_first;  // pooling backward is the first write into the source's gradient
   long long nparam = 0;
   float *p = nullptr, *gp = nullptr, *m = nullptr, *v = nullptr;  // params, grads, Adam moments
   long long adam_t = 0;
@@ -856,8 +885,8 @@ struct lbm_sur {
   long long gup_cap = 0;
   float *grad_arena = nullptr;  // every Tensor::grad, B samples each
   long long grad_n = 0;
-  u16 *gT = nullptr, *xT = nullptr;  // channel-major operands of k_wgrad_t
-  float *bpart = nullptr;            // bias partial sums of k_gy_prep
+  u16 *gT = nullptr, *xT = nullptr;  // channel-major operands of k_wgrad_t (gT: two buffers)
+  float *bpart = nullptr;            // bias partial sums of k_gy_prep (two buffers)
   long long gT_cap = 0, xT_cap = 0, bpart_cap = 0;
   double *lpart = nullptr, *lsum = nullptr;
   // dataset
@@ -866,7 +895,14 @@ struct lbm_sur {
   long long dn = 0;
   std::vector<long long> dfluid;
   hipStream_t st = nullptr;
+  // the weight gradients run on a second stream beside the input-gradient
+  // chain: wg waits for ev_gy[k] (gT/bpart buffer k written), st for ev_wg[k]
+  // (buffer k read) before writing it again, and for ev_wg of the last one
+  // before the backward pass ends
+  hipStream_t wg = nullptr;
+  hipEvent_t ev_fwd = nullptr, ev_gy[2] = {nullptr, nullptr}, ev_wg[2] = {nullptr, nullptr};
   std::string err;
+  bool ok_order = true;  // every gradient is written before it is read
 };
 
 static thread_local std::string g_sur_err;
@@ -932,6 +968,31 @@ static void build_net(lbm_sur *s) {
   }
   for (int lv = NLEV - 2; lv >= 0; --lv) cur = conv(-1, conv(cur, enc[lv], WIDTH[lv], true), WIDTH[lv], true);
   conv(-1, cur, 32, false, true);  // 3 used outputs, padded to one 32-channel tile
+  // which write into each tensor gradient comes first in backward order
+  // (the order of loss_backward): that one stores, the later ones add, so no
+  // gradient is cleared before the backward pass; every gradient a layer
+  // reads (its output's) has been written by then
+  std::vector<char> seen(s->t.size(), 0);
+  s->pool_first.assign(s->pool_pairs.size(), 0);
+  size_t pi = s->pool_pairs.size();
+  for (int li = (int)s->L.size() - 1; li >= 0; --li) {
+    Layer &l = s->L[li];
+    if (!l.last && !seen[l.out]) s->ok_order = false;
+    if (li > 0) {
+      l.s_first = !seen[l.s];
+      seen[l.s] = 1;
+      if (l.a >= 0) {
+        l.a_first = !seen[l.a];
+        seen[l.a] = 1;
+      }
+    }
+    while (pi > 0 && s->pool_pairs[pi - 1].second == l.s) {
+      --pi;
+      const int src = s->pool_pairs[pi].first;
+      s->pool_first[pi] = !seen[src];
+      seen[src] = 1;
+    }
+  }
   long long off = 0;
   for (Layer &l : s->L) {
     l.woff = off;
@@ -984,7 +1045,8 @@ static int alloc_all(lbm_sur *s) {
       s->bpart_cap = std::max(s->bpart_cap, (np + TP - 1) / TP * l.co);
     }
   }
-  // one arena for every tensor's gradient: cleared by one launch per step
+  // one arena for every tensor's gradient (each written by its first writer
+  // in backward order before anything reads it)
   SCHK(s, hipMalloc(&s->grad_arena, s->grad_n * sizeof(float)));
   {
     long long off = 0;
@@ -1005,9 +1067,9 @@ static int alloc_all(lbm_sur *s) {
   SCHK(s, hipMalloc(&s->gout, np0 * 32 * sizeof(u16)));
   SCHK(s, hipMalloc(&s->part, s->part_cap * sizeof(float)));
   SCHK(s, hipMalloc(&s->gup, std::max(1LL, s->gup_cap) * sizeof(float)));
-  SCHK(s, hipMalloc(&s->gT, std::max(1LL, s->gT_cap) * sizeof(u16)));
+  SCHK(s, hipMalloc(&s->gT, 2 * std::max(1LL, s->gT_cap) * sizeof(u16)));
   SCHK(s, hipMalloc(&s->xT, std::max(1LL, s->xT_cap) * sizeof(u16)));
-  SCHK(s, hipMalloc(&s->bpart, std::max(1LL, s->bpart_cap) * sizeof(float)));
+  SCHK(s, hipMalloc(&s->bpart, 2 * std::max(1LL, s->bpart_cap) * sizeof(float)));
   SCHK(s, hipMalloc(&s->lpart, ((np0 + 255) / 256) * sizeof(double)));
   SCHK(s, hipMalloc(&s->lsum, sizeof(double)));
   return LBM_OK;
@@ -1117,8 +1179,12 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, s->st, (const double *)s->lpart, nbl, s->lsum);
   SCHK(s, hipGetLastError());
   if (!grads) return LBM_OK;
-  hipLaunchKernelGGL(k_fill, dim3(nblk(s->grad_n)), dim3(256), 0, s->st, s->grad_arena, s->grad_n, 0.f);
   size_t pi = s->pool_pairs.size();
+  // the activations of the forward pass are read on the weight-gradient stream
+  SCHK(s, hipEventRecord(s->ev_fwd, s->st));
+  SCHK(s, hipStreamWaitEvent(s->wg, s->ev_fwd, 0));
+  bool pending[2] = {false, false};
+  int ntr = 0, last = -1;
   for (int li = (int)s->L.size() - 1; li >= 0; --li) {
     const Layer &l = s->L[li];
     const Tensor &S = s->t[l.s];
@@ -1133,14 +1199,20 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
     if (wgrad_transposed(l, w)) {
       const Tensor &O = s->t[l.last ? l.s : l.out];
       const unsigned ptiles = (unsigned)((np + TP - 1) / TP);
+      const int k = ntr++ & 1;
+      u16 *gT = s->gT + k * std::max(1LL, s->gT_cap);
+      float *bpart = s->bpart + k * std::max(1LL, s->bpart_cap);
+      if (pending[k]) SCHK(s, hipStreamWaitEvent(s->st, s->ev_wg[k], 0));
       hipLaunchKernelGGL(k_gy_prep, dim3(ptiles, l.co / 32), dim3(256), 0, s->st, l.last ? nullptr : (const float *)O.grad,
                          l.last ? nullptr : (const u16 *)O.act, (!l.last && O.relu) ? 1 : 0,
-                         l.last ? (const u16 *)s->gout : nullptr, s->gout, s->gT, s->bpart, np, l.co);
+                         l.last ? (const u16 *)s->gout : nullptr, s->gout, gT, bpart, np, l.co);
+      SCHK(s, hipEventRecord(s->ev_gy[k], s->st));
       hipLaunchKernelGGL(k_x_prep, dim3((unsigned)((w + 8 + TP - 1) / TP), (unsigned)(nb * (h + 2)), (unsigned)((ci + 31) / 32)),
-                         dim3(256), 0, s->st, l.a >= 0 ? (const u16 *)s->t[l.a].act : nullptr, l.ca,
+                         dim3(256), 0, s->wg, l.a >= 0 ? (const u16 *)s->t[l.a].act : nullptr, l.ca,
                          (const u16 *)S.act, l.cs, nb, h, w, s->xT);
+      SCHK(s, hipStreamWaitEvent(s->wg, s->ev_gy[k], 0));
       WgradTArgs wa;
-      wa.gT = s->gT;
+      wa.gT = gT;
       wa.xT = s->xT;
       wa.P = np;
       wa.co = l.co;
@@ -1152,12 +1224,15 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
       wa.nchunks = (int)((np / 8 + wa.cseg - 1) / wa.cseg);
       wa.part = s->part;
       const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 3 * wa.nchunks;
-      hipLaunchKernelGGL(k_wgrad_t, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->st, wa);
-      hipLaunchKernelGGL(k_reduce_chunks, dim3(nblk32(nw)), dim3(256), 0, s->st, (const float *)s->part, nw, wa.nchunks,
-                         s->gp + l.woff);
-      hipLaunchKernelGGL(k_reduce_rows, dim3(l.co), dim3(256), 0, s->st, (const float *)s->bpart, (long long)ptiles, l.co,
-                         s->gp + l.boff);
+      hipLaunchKernelGGL(k_wgrad_t, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->wg, wa);
+      hipLaunchKernelGGL(k_reduce_wb, dim3(nblk32(nw) + l.co), dim3(256), 0, s->wg, (const float *)s->part, nw,
+                         wa.nchunks, s->gp + l.woff, (const float *)bpart, (long long)ptiles, l.co, s->gp + l.boff);
+      SCHK(s, hipEventRecord(s->ev_wg[k], s->wg));
+      pending[k] = true;
+      last = k;
     } else {
+      // s->part is the weight-gradient stream's too
+      if (last >= 0) SCHK(s, hipStreamWaitEvent(s->st, s->ev_wg[last], 0));
       if (!l.last) {
         Tensor &O = s->t[l.out];
         hipLaunchKernelGGL(k_mask_cast, dim3(nblk(np * l.co)), dim3(256), 0, s->st, (const float *)O.grad,
@@ -1198,13 +1273,12 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
       a.c0 = l.ca;
       a.g0 = l.ca > 0 ? s->gup : nullptr;
       a.g1 = S.grad;
-      if (l.ca > 0)
-        hipLaunchKernelGGL(k_fill, dim3(nblk(np * l.ca)), dim3(256), 0, s->st, s->gup, np * l.ca, 0.f);
+      a.add1 = l.s_first ? 0 : 1;
       launch_conv(1, a, s->st);
       if (l.ca > 0) {
         Tensor &A = s->t[l.a];
         hipLaunchKernelGGL(k_up_bwd, dim3(nblk(np / 4 * l.ca)), dim3(256), 0, s->st, (const float *)s->gup, A.grad,
-                           nb, h / 2, w / 2, l.ca);
+                           nb, h / 2, w / 2, l.ca, l.a_first ? 0 : 1);
       }
     }
     // pooling steps that produced this layer's input run backward next
@@ -1214,9 +1288,10 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
       const Tensor &dst = s->t[s->pool_pairs[pi].second];
       const int hh = s->H >> dst.level, ww = s->W >> dst.level;
       hipLaunchKernelGGL(k_pool_bwd, dim3(nblk((long long)nb * 4 * hh * ww * dst.c)), dim3(256), 0, s->st,
-                         (const float *)dst.grad, src.grad, nb, hh, ww, dst.c);
+                         (const float *)dst.grad, src.grad, nb, hh, ww, dst.c, s->pool_first[pi] ? 0 : 1);
     }
   }
+  if (last >= 0) SCHK(s, hipStreamWaitEvent(s->st, s->ev_wg[last], 0));
   SCHK(s, hipGetLastError());
   return LBM_OK;
 }
@@ -1242,8 +1317,13 @@ int lbm_sur_create(int32_t batch, int32_t height, int32_t width, int32_t in_chan
     lbm_sur_destroy(s);
     return rc;
   };
-  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess)
+  if (!s->ok_order) return bail(sfail(nullptr, LBM_EINVAL, "lbm_sur_create: a gradient is read before it is written"));
+  if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->wg, hipStreamNonBlocking) != hipSuccess)
     return bail(sfail(nullptr, LBM_EHIP, "lbm_sur_create: stream"));
+  for (hipEvent_t *e : {&s->ev_fwd, &s->ev_gy[0], &s->ev_gy[1], &s->ev_wg[0], &s->ev_wg[1]})
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess)
+      return bail(sfail(nullptr, LBM_EHIP, "lbm_sur_create: event"));
   int rc = alloc_all(s);
   if (rc) return bail(sfail(nullptr, rc, s->err));
   // He-normal weights, zero biases; the output layer starts 10x smaller
@@ -1435,6 +1515,7 @@ const char *lbm_sur_last_error(const lbm_sur *s) { return s ? s->err.c_str() : g
 int lbm_sur_destroy(lbm_sur *s) {
   if (!s) return LBM_OK;
   if (s->st) (void)hipStreamSynchronize(s->st);
+  if (s->wg) (void)hipStreamSynchronize(s->wg);
   for (Tensor &T : s->t) {
     if (T.act) (void)hipFree(T.act);
     T.grad = nullptr;  // in the arena
@@ -1447,7 +1528,10 @@ int lbm_sur_destroy(lbm_sur *s) {
                   s->gT, s->xT, s->bpart, s->grad_arena, s->lpart, s->lsum, s->dx, s->dt, s->dm};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  for (hipEvent_t e : {s->ev_fwd, s->ev_gy[0], s->ev_gy[1], s->ev_wg[0], s->ev_wg[1]})
+    if (e) (void)hipEventDestroy(e);
   if (s->st) (void)hipStreamDestroy(s->st);
+  if (s->wg) (void)hipStreamDestroy(s->wg);
   delete s;
   return LBM_OK;
 }

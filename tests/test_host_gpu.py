@@ -30,10 +30,13 @@ def run_cli(cfg, tmp_path, *args, timeout=300):
     return r
 
 
-def test_couette_reference_thresholds_f64(tmp_path):
-    """test/couette/couette.json: periodic x, moving top wall (tangentialVelocity),
-    reynoldsnumber-based omega; must pass its own errorMax 4e-11 / errorL2 7e-10."""
-    cfg = json.load(open(os.path.join(REFCFG, "couette__couette.json")))
+@pytest.mark.parametrize("name", ["couette__couette.json", "couette__couette_bnd.json"])
+def test_couette_reference_thresholds_f64(name, tmp_path):
+    """test/couette/couette.json (periodic links) and couette_bnd.json
+    (periodic boundary objects that copy, generateBndry = true): moving top
+    wall (tangentialVelocity), reynoldsnumber-based omega; each must pass
+    its own errorMax 4e-11 / errorL2 7e-10, unmodified."""
+    cfg = json.load(open(os.path.join(REFCFG, name)))
     r = run_cli(cfg, tmp_path, "-p", "f64")
     print(r.stderr[-3000:])
     assert r.returncode == 0, r.stderr[-2000:]
