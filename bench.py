@@ -100,7 +100,7 @@ def run_batch(args):
     doms = W.backward_step_batch(n=args.ndom, seed=2024)
     b = capi.Batch(doms, storage=args.storage)
     t0 = time.perf_counter()
-    tx, blocks, tms = b.tune(10)
+    tx, blocks, one, tms = b.tune(10)
     tune_s = time.perf_counter() - t0
     rho, u = zip(*[W.plug_start(d, 0.03) for d in doms])
     b.init(np.concatenate(rho), np.concatenate(u))
@@ -132,7 +132,8 @@ def run_batch(args):
             "steps_max": int(steps.max()), "updates": updates, "GLUPS_effective": round(glups, 2),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(achieved / PEAK_HBM_GBS, 4)},
-            "tiling": {"tile_x": tx, "blocks": blocks, "tuning_s": round(tune_s, 2), "ms_per_step_full": tms},
+            "tiling": {"tile_x": tx, "blocks": blocks, "one_launch": one, "tuning_s": round(tune_s, 2),
+                       "ms_per_step_full": tms},
             "tiles": info["tiles"]}
     print(json.dumps(line), flush=True)
 

@@ -40,14 +40,24 @@ typedef struct {
 
 int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *doms, int32_t ndom, lbm_batch **out);
 
-/* Autotuner: times every candidate tiling (tile width 32/64/128/256 cells,
- * persistent grids of 2..16 blocks per CU or one block per tile) for
- * steps_per_trial steps on the batch and keeps the fastest.  Call it before
+/* Autotuner: times every candidate tiling (2D tiles of 256 cells and width
+ * 16/32/64/128/256, or tile_x 0: row runs, 256 consecutive cells of a
+ * rectangle of cells of one class -- interior, boundary, next to a solid;
+ * each wave of 64 cells takes the boundary paths only when one of its cells
+ * needs them; persistent grids of 2..16 blocks per CU or one block per tile)
+ * and both launch forms (lbm_batch_set_launch) for steps_per_trial steps
+ * on the batch and keeps the fastest.  Call it before
  * lbm_batch_init_fields (it advances whatever state the buffers hold).
- * Writes "tile_x,blocks" of the choice into choice (may be NULL). */
+ * Writes "tile_x,blocks,one_launch,ms_per_step" of the choice into choice
+ * (may be NULL). */
 int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t choice_len);
 /* Apply a tiling directly (for example one the autotuner chose earlier). */
 int lbm_batch_set_tiling(lbm_batch *b, int32_t tile_x, int32_t blocks);
+/* Launch form of a step: one_launch 1 (the default), one kernel over every
+ * tile, each wave taking its class's path; 0, two kernels: the tiles whose
+ * waves are all interior (compiled without the boundary code), then the
+ * others. */
+int lbm_batch_set_launch(lbm_batch *b, int32_t one_launch);
 
 /* rho and u of every domain concatenated in domain order (compact,
  * x fastest); u cell-major [cell][dim]. */

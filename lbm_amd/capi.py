@@ -279,7 +279,8 @@ class DomainResult(ct.Structure):
     _fields_ = [("steps", ct.c_int64), ("res_rel", ct.c_double), ("mass", ct.c_double)]
 
 
-BATCH_EXPORTS = ["lbm_batch_create", "lbm_batch_tune", "lbm_batch_set_tiling", "lbm_batch_init_fields",
+BATCH_EXPORTS = ["lbm_batch_create", "lbm_batch_tune", "lbm_batch_set_tiling", "lbm_batch_set_launch",
+                 "lbm_batch_init_fields",
                  "lbm_batch_run", "lbm_batch_get_macros", "lbm_batch_get_populations", "lbm_batch_info",
                  "lbm_batch_last_error", "lbm_batch_destroy"]
 _batch_ready = False
@@ -295,6 +296,7 @@ def _load_batch():
     lib.lbm_batch_create.argtypes = [ct.c_int32, ct.c_int32, ct.POINTER(DomainDesc), ct.c_int32, ct.POINTER(P)]
     lib.lbm_batch_tune.argtypes = [P, ct.c_int32, ct.c_char_p, ct.c_int32]
     lib.lbm_batch_set_tiling.argtypes = [P, ct.c_int32, ct.c_int32]
+    lib.lbm_batch_set_launch.argtypes = [P, ct.c_int32]
     lib.lbm_batch_init_fields.argtypes = [P, dp, dp]
     lib.lbm_batch_run.argtypes = [P, ct.c_int32, ct.c_double, ct.c_int64, ct.POINTER(DomainResult), dp]
     lib.lbm_batch_get_macros.argtypes = [P, ct.c_int32, dp, dp]
@@ -357,13 +359,18 @@ class Batch:
             raise LBMError(rc, self.lib.lbm_batch_last_error(self.h).decode())
 
     def tune(self, steps=20):
+        """Times every tiling and launch form; returns (tile_x, blocks,
+        one_launch, ms per step) of the fastest, which stays set."""
         buf = ct.create_string_buffer(64)
         self._check(self.lib.lbm_batch_tune(self.h, int(steps), buf, 64))
-        tx, blocks, ms = buf.value.decode().split(",")
-        return int(tx), int(blocks), float(ms)
+        tx, blocks, one, ms = buf.value.decode().split(",")
+        return int(tx), int(blocks), int(one), float(ms)
 
     def set_tiling(self, tx, blocks):
         self._check(self.lib.lbm_batch_set_tiling(self.h, int(tx), int(blocks)))
+
+    def set_launch(self, one_launch):
+        self._check(self.lib.lbm_batch_set_launch(self.h, int(one_launch)))
 
     def init(self, rho, u):
         rho = np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1))

@@ -70,12 +70,16 @@ struct lbm_batch {
   uint8_t *flags = nullptr;
   uint32_t *links = nullptr;  // solid-link masks, same layout as flags
   DomainDev *doms_d = nullptr;
-  int4 *tiles_d = nullptr;
+  BatchTile *tiles_d = nullptr;
   long long ntiles = 0;
-  int *ids_d = nullptr;  // interior tile indices, then the others
+  std::vector<std::vector<uint8_t>> cclass;       // every domain's cell classes (tiles)
+  std::vector<std::vector<BatchTile>> dom_tiles;  // every domain's tiles for tiling cache_tx
+  int cache_tx = -1;
+  int *ids_d = nullptr;  // launch lists: interior tiles, boundary tiles, every tile
   int n_int = 0, n_bnd = 0;
   int tx = 64, blocks = 0;  // blocks == 0: one block per tile
   int xcd_chunks = 1;       // LBM_BATCH_XCD=0 turns the per-XCD tile ranges off
+  int one_launch = 1;       // 1: one launch over every tile; 0: interior tiles, then the others
   int *active_d = nullptr;
   double *res_d = nullptr, *mass_d = nullptr, *partials = nullptr;
   long long *conv_d = nullptr, *cycle_d = nullptr;
@@ -111,57 +115,196 @@ static void batch_drop_graph(lbm_batch *b) {
   }
 }
 
-// Tiles of every still-active domain.  A tile needs the flag field when a
-// solid cell lies within one cell of it (its pull sources or itself).  The
-// new buffers are allocated and filled before the old ones are freed, so a
-// failed rebuild leaves the previous tiling in place (without a graph; the
-// next run captures a new one).
+// Tiles of one domain.  Every cell is classified first: solid (never
+// updated), 0 interior (no face, wrap or solid within one cell: the path
+// without boundary code), 1 boundary, 2 boundary with a solid within one
+// cell (flag loads; solids across periodic faces count).  A wave takes the
+// path of the highest class among its fluid cells (tile field cls, two bits
+// per wave), class 2 when it also holds a solid cell, and 3 (no update)
+// when it holds none of the domain's fluid cells.  tx > 0: 2D tiles of tx x (256 / tx) cells in one plane (tiles
+// of solid cells only are left out).  tx == 0: rows: each row is cut into
+// runs of one class, runs that repeat over consecutive rows become
+// rectangles, and each rectangle is cut into linear pieces of 256 cells.
+constexpr uint8_t CELL_SOLID = 255;
+static void cell_classes(const DomainDev &D, const std::vector<uint8_t> &sol, std::vector<uint8_t> &cc) {
+  const int nx = D.nx, ny = D.ny, nz = D.nz;
+  // periodic axes of the lattice (a 2D domain's z faces carry the periodic
+  // default but never wrap)
+  const bool per3[3] = {D.per[0] != 0, D.per[1] != 0, nz > 1 && D.per[2] != 0};
+  auto at = [&](int x, int y, int z) { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z); };
+  // near[c]: a solid cell within one cell of c, across periodic faces too
+  // (box dilation, axis by axis)
+  std::vector<uint8_t> near;
+  if (!sol.empty()) {
+    near.resize(sol.size());
+    for (size_t c = 0; c < sol.size(); ++c) near[c] = sol[c] != 0;
+    std::vector<uint8_t> tmp(near.size());
+    const int n3[3] = {nx, ny, nz};
+    const size_t st3[3] = {1, (size_t)nx, (size_t)nx * ny};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (n3[ax] == 1) continue;
+      const size_t st = st3[ax];
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const int k = ax == 0 ? x : (ax == 1 ? y : z);
+            const size_t c = at(x, y, z);
+            const size_t span = (size_t)(n3[ax] - 1) * st;
+            uint8_t v = near[c];
+            if (k > 0) v |= near[c - st];
+            else if (per3[ax]) v |= near[c + span];
+            if (k < n3[ax] - 1) v |= near[c + st];
+            else if (per3[ax]) v |= near[c - span];
+            tmp[c] = v;
+          }
+      near.swap(tmp);
+    }
+  }
+  cc.assign((size_t)nx * ny * nz, 0);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t c = at(x, y, z);
+        uint8_t k;
+        if (!sol.empty() && sol[c]) k = CELL_SOLID;
+        else if (!sol.empty() && near[c]) k = 2;
+        else k = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && (nz == 1 || (z >= 1 && z <= nz - 2))) ? 0 : 1;
+        cc[c] = k;
+      }
+}
+
+static void domain_tiles(const DomainDev &D, const std::vector<uint8_t> &cc, int d, int tx,
+                         std::vector<BatchTile> &out) {
+  const int nx = D.nx, ny = D.ny, nz = D.nz;
+  constexpr uint8_t SOLID = CELL_SOLID;
+  auto at = [&](int x, int y, int z) { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z); };
+  if (tx > 0) {
+    const int ty = 256 / tx;
+    for (int z = 0; z < nz; ++z)
+      for (int y0 = 0; y0 < ny; y0 += ty)
+        for (int x0 = 0; x0 < nx; x0 += tx) {
+          int mask = 0;
+          bool fluid = false;
+          for (int w = 0; w < 4; ++w) {
+            // the path without boundary code never sees a solid lane: it
+            // would update the cell as fluid and, on a domain edge, pull
+            // from outside the domain
+            int wc = 0;
+            bool wfluid = false, wsolid = false;
+            for (int i = 64 * w; i < 64 * w + 64; ++i) {
+              const int x = x0 + i % tx, y = y0 + i / tx;
+              if (x >= nx || y >= ny) continue;
+              const uint8_t k = cc[at(x, y, z)];
+              if (k == SOLID) {
+                wsolid = true;
+                continue;
+              }
+              wfluid = true;
+              wc = std::max(wc, (int)k);
+            }
+            if (!wfluid) wc = 3;            // nothing to update: the wave skips the cell update
+            else if (wsolid) wc = 2;        // flag loads deactivate the solid lanes
+            fluid = fluid || wfluid;
+            mask |= wc << (2 * w);
+          }
+          if (fluid) out.push_back(BatchTile{d, x0, tx, y0, 0, 256, z, mask});
+        }
+    return;
+  }
+  struct Run {
+    int xa, xb, cls, ya;
+  };
+  struct Rect {
+    int xa, xb, ya, yb, cls;
+  };
+  std::vector<Run> open, next, row;
+  std::vector<Rect> rects;
+  for (int z = 0; z < nz; ++z) {
+    open.clear();
+    rects.clear();
+    for (int y = 0; y <= ny; ++y) {
+      row.clear();
+      if (y < ny)
+        for (int x = 0; x < nx;) {
+          auto cls = [&](int xx) {
+            const uint8_t k = cc[at(xx, y, z)];
+            return k == SOLID ? -1 : (int)k;
+          };
+          const int k = cls(x);
+          int xe = x + 1;
+          while (xe < nx && cls(xe) == k) ++xe;
+          if (k >= 0) row.push_back(Run{x, xe, k, y});
+          x = xe;
+        }
+      // continue the open rectangles whose run repeats in this row (both
+      // lists are ordered by xa), close the others
+      next.clear();
+      size_t i = 0;
+      for (const Run &r : row) {
+        while (i < open.size() && open[i].xa < r.xa) {
+          rects.push_back(Rect{open[i].xa, open[i].xb, open[i].ya, y, open[i].cls});
+          ++i;
+        }
+        if (i < open.size() && open[i].xa == r.xa && open[i].xb == r.xb && open[i].cls == r.cls) {
+          next.push_back(open[i]);
+          ++i;
+        } else {
+          next.push_back(r);
+        }
+      }
+      for (; i < open.size(); ++i) rects.push_back(Rect{open[i].xa, open[i].xb, open[i].ya, y, open[i].cls});
+      open.swap(next);
+    }
+    std::sort(rects.begin(), rects.end(),
+              [](const Rect &a, const Rect &b) { return a.ya != b.ya ? a.ya < b.ya : a.xa < b.xa; });
+    for (const Rect &r : rects) {
+      const int w = r.xb - r.xa, total = w * (r.yb - r.ya);
+      for (int s0 = 0; s0 < total; s0 += 256)
+        out.push_back(BatchTile{d, r.xa, w, r.ya, s0, std::min(256, total - s0), z, r.cls * 0x55});
+    }
+  }
+}
+
+// Tiles of every still-active domain (each domain's list is built once per
+// tiling and kept).  The new buffers are allocated and filled before the
+// old ones are freed, so a failed rebuild leaves the previous tiling in
+// place (without a graph; the next run captures a new one).
 static int batch_build_tiles(lbm_batch *b, int tx) {
-  const int ty = 256 / tx;
   BHIPCHK(b, hipStreamSynchronize(b->stream));
   batch_drop_graph(b);
-  std::vector<int4> tiles;
+  if ((int)b->cclass.size() != b->ndom) {
+    b->cclass.assign(b->ndom, {});
+    for (int d = 0; d < b->ndom; ++d) cell_classes(b->doms[d], b->solid[d], b->cclass[d]);
+  }
+  if (b->cache_tx != tx || (int)b->dom_tiles.size() != b->ndom) {
+    b->dom_tiles.assign(b->ndom, {});
+    for (int d = 0; d < b->ndom; ++d) domain_tiles(b->doms[d], b->cclass[d], d, tx, b->dom_tiles[d]);
+    b->cache_tx = tx;
+  }
+  std::vector<BatchTile> tiles;
   std::vector<DomainDev> doms = b->doms;
-  // interior tiles: full, no face, wrap or solid within one cell of them;
-  // they run in a kernel compiled without the boundary paths
+  // interior tiles run in a kernel compiled without the boundary paths
   std::vector<int> ids_int, ids_bnd;
   for (int d = 0; d < b->ndom; ++d) {
     DomainDev &D = doms[d];
     D.tile_begin = (int)tiles.size();
-    if (!b->active_h.empty() && !b->active_h[d]) {
-      D.tile_end = D.tile_begin;
-      continue;
-    }
-    const std::vector<uint8_t> &sol = b->solid[d];
-    for (int z = 0; z < D.nz; ++z)
-      for (int y0 = 0; y0 < D.ny; y0 += ty)
-        for (int x0 = 0; x0 < D.nx; x0 += tx) {
-          int need = 0;
-          if (!sol.empty()) {
-            for (int zz = std::max(0, z - 1); zz <= std::min(D.nz - 1, z + 1) && !need; ++zz)
-              for (int yy = std::max(0, y0 - 1); yy <= std::min(D.ny - 1, y0 + ty) && !need; ++yy)
-                for (int xx = std::max(0, x0 - 1); xx <= std::min(D.nx - 1, x0 + tx); ++xx)
-                  if (sol[xx + (size_t)D.nx * (yy + (size_t)D.ny * zz)]) {
-                    need = 1;
-                    break;
-                  }
-          }
-          const bool interior = !need && x0 >= 1 && x0 + tx <= D.nx - 1 && y0 >= 1 && y0 + ty <= D.ny - 1 &&
-                                (D.nz == 1 || (z >= 1 && z <= D.nz - 2));
-          // periodic wrap reaches the opposite edge
-          if (!sol.empty() && !need && (D.per[0] || D.per[1] || D.per[2])) need = 1;
-          (interior ? ids_int : ids_bnd).push_back((int)tiles.size());
-          tiles.push_back(make_int4(d, x0, y0, z | (need << 30)));
-        }
+    if (b->active_h.empty() || b->active_h[d])
+      for (const BatchTile &t : b->dom_tiles[d]) {
+        (t.cls == 0 ? ids_int : ids_bnd).push_back((int)tiles.size());  // every wave interior, or not
+        tiles.push_back(t);
+      }
     D.tile_end = (int)tiles.size();
   }
   if (tiles.empty()) {
     ids_bnd.push_back(0);
-    tiles.push_back(make_int4(0, 0, 0, 0));
+    tiles.push_back(BatchTile{0, 0, 1, 0, 0, 0, 0, 1});  // no lane active
   }
+  // launch lists: the interior tiles, the boundary tiles, then every tile in
+  // list order (the single-launch form)
   std::vector<int> ids(ids_int);
   ids.insert(ids.end(), ids_bnd.begin(), ids_bnd.end());
-  int4 *tiles_d = nullptr;
+  for (int t = 0; t < (int)tiles.size(); ++t) ids.push_back(t);
+  BatchTile *tiles_d = nullptr;
   int *ids_d = nullptr;
   double *partials = nullptr;
   auto release = [&](int code, const std::string &msg) {
@@ -170,11 +313,11 @@ static int batch_build_tiles(lbm_batch *b, int tx) {
     if (partials) (void)hipFree(partials);
     return bfail(b, code, msg);
   };
-  if (hipMalloc(&tiles_d, tiles.size() * sizeof(int4)) != hipSuccess ||
+  if (hipMalloc(&tiles_d, tiles.size() * sizeof(BatchTile)) != hipSuccess ||
       hipMalloc(&ids_d, ids.size() * sizeof(int)) != hipSuccess ||
       hipMalloc(&partials, tiles.size() * NSUM * sizeof(double)) != hipSuccess)
     return release(LBM_ENOMEM, "batch tiling: device allocation failed");
-  if (hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
+  if (hipMemcpy(tiles_d, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(ids_d, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(b->doms_d, doms.data(), doms.size() * sizeof(DomainDev), hipMemcpyHostToDevice) != hipSuccess)
     return release(LBM_EHIP, "batch tiling: upload failed");
@@ -201,9 +344,8 @@ static BatchArgs batch_args(lbm_batch *b, int parity, int mode) {
   a.doms = b->doms_d;
   a.tiles = b->tiles_d;
   a.ntiles = (int)b->ntiles;
-  a.tx = b->tx;
-  a.xcd_chunks = b->xcd_chunks;
   a.active = b->active_d;
+  a.xcd_chunks = b->xcd_chunks;
   if (mode >= 1) {
     // snapshot written by the step before the check, read by the check
     a.snap_in = b->snap[0];
@@ -223,15 +365,23 @@ static int batch_grid(const lbm_batch *b, int n) {
 
 static void batch_launch(lbm_batch *b, int parity, int mode) {
   BatchArgs a = batch_args(b, parity, mode);
+  if (b->one_launch) {
+    // every tile in list order (domain-major, rows in order), so a tile and
+    // its neighbours of either class share the L2 of their XCD
+    a.ids = b->ids_d + b->n_int + b->n_bnd;
+    a.nids = (int)b->ntiles;
+    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, (int)b->ntiles), b->stream, mode, 2);
+    return;
+  }
   if (b->n_int > 0) {
     a.ids = b->ids_d;
     a.nids = b->n_int;
-    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_int), b->stream, mode, true);
+    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_int), b->stream, mode, 1);
   }
   if (b->n_bnd > 0) {
     a.ids = b->ids_d + b->n_int;
     a.nids = b->n_bnd;
-    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_bnd), b->stream, mode, false);
+    dispatch_qs<BatchStepL>(b->Q, b->storage, a, batch_grid(b, b->n_bnd), b->stream, mode, 2);
   }
 }
 
@@ -360,10 +510,19 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
 }
 
 int lbm_batch_set_tiling(lbm_batch *b, int32_t tile_x, int32_t blocks) {
-  if (!b || !(tile_x == 32 || tile_x == 64 || tile_x == 128 || tile_x == 256) || blocks < 0)
-    return bfail(b, LBM_EINVAL, "lbm_batch_set_tiling: tile_x in {32,64,128,256}, blocks >= 0");
+  if (!b || !(tile_x == 0 || tile_x == 16 || tile_x == 32 || tile_x == 64 || tile_x == 128 || tile_x == 256) ||
+      blocks < 0)
+    return bfail(b, LBM_EINVAL, "lbm_batch_set_tiling: tile_x in {0,16,32,64,128,256}, blocks >= 0");
   b->blocks = blocks;
   return batch_build_tiles(b, tile_x);
+}
+
+int lbm_batch_set_launch(lbm_batch *b, int32_t one_launch) {
+  if (!b || one_launch < 0 || one_launch > 1) return bfail(b, LBM_EINVAL, "lbm_batch_set_launch: one_launch 0 or 1");
+  BHIPCHK(b, hipStreamSynchronize(b->stream));
+  batch_drop_graph(b);  // the next run captures the new form
+  b->one_launch = one_launch;
+  return LBM_OK;
 }
 
 int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t choice_len) {
@@ -377,14 +536,15 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
   BHIPCHK(b, ev.create());
   hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   double best = 1e30;
-  int best_tx = 64, best_blocks = 0;
-  const int txs[4] = {32, 64, 128, 256};
+  int best_tx = 64, best_blocks = 0, best_one = 1;
+  const int txs[6] = {0, 16, 32, 64, 128, 256};
   const int per_cu[4] = {0, 2, 4, 16};
-  for (int ti = 0; ti < 4; ++ti) {
+  for (int ti = 0; ti < 6; ++ti) {
     int rc = batch_build_tiles(b, txs[ti]);
     if (rc) return rc;
-    for (int gi = 0; gi < 4; ++gi) {
-      b->blocks = per_cu[gi] * ncu;
+    for (int gi = 0; gi < 8; ++gi) {
+      b->blocks = per_cu[gi % 4] * ncu;
+      b->one_launch = gi < 4 ? 1 : 0;
       for (int w = 0; w < 2; ++w) batch_launch(b, (b->cur + w) & 1, 0);
       BHIPCHK(b, hipEventRecord(e0, b->stream));
       for (int s = 0; s < steps_per_trial; ++s) batch_launch(b, (b->cur + s) & 1, 0);
@@ -396,13 +556,16 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
         best = t;
         best_tx = txs[ti];
         best_blocks = b->blocks;
+        best_one = b->one_launch;
       }
     }
   }
   b->blocks = best_blocks;
+  b->one_launch = best_one;
   int rc = batch_build_tiles(b, best_tx);
   if (rc) return rc;
-  if (choice && choice_len > 0) snprintf(choice, choice_len, "%d,%d,%.4f", best_tx, best_blocks, best / steps_per_trial);
+  if (choice && choice_len > 0)
+    snprintf(choice, choice_len, "%d,%d,%d,%.4f", best_tx, best_blocks, best_one, best / steps_per_trial);
   return LBM_OK;
 }
 

@@ -23,6 +23,7 @@
 // one-step change of (u, rho) in FP64 with a fixed-order two-stage tree.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>

@@ -21,6 +21,19 @@ struct DomainDev {
   double omega;
 };
 
+// One tile of a batch: up to 256 cells of domain d, the linear range
+// [s, s + n) of the row-major rectangle of width w whose first cell is
+// (xs, ys) in plane z; lane i updates cell s + i.  Bits 2w, 2w + 1 of cls:
+// the path of wave w, 0: no face, wrap or solid within one cell of its
+// cells (no boundary code); 1: boundary paths without flag loads; 2:
+// boundary paths and flag loads (a solid within one cell, or a solid
+// lane); 3: no fluid lane (no update).  A 2D tile of
+// tx x (256 / tx) cells is the rectangle of width tx at (x0, y0) with
+// s = 0, n = 256 (lanes outside the domain are idle).
+struct BatchTile {
+  int d, xs, w, ys, s, n, z, cls;
+};
+
 // G for cell_update: one domain of a batch
 struct BatchView {
   const void *src;
@@ -41,9 +54,8 @@ struct BatchArgs {
   void *dst;
   const uint32_t *links;  // solid-link masks of all domains (cell_off layout)
   const DomainDev *doms;
-  const int4 *tiles;
+  const BatchTile *tiles;
   int ntiles;
-  int tx;  // tile width in cells (tile = tx x (256/tx))
   int xcd_chunks;  // 1: each XCD walks a contiguous range of tiles
   const int *ids;  // the tiles this launch covers (indices into tiles)
   int nids;
@@ -64,9 +76,10 @@ struct StepL {
 
 template <int Q, typename ST>
 struct BatchStepL {
-  // mode: 0 plain, 1 snapshot, 2 check; interior: the tiles in a.ids have
-  // no boundary within one cell (compiled without the boundary paths)
-  static void run(const BatchArgs &a, int grid, hipStream_t s, int mode, bool interior);
+  // mode: 0 plain, 1 snapshot, 2 check; path 1: every wave of the tiles in
+  // a.ids is interior (compiled without the boundary paths); 2: any tiles,
+  // each wave branching on its class
+  static void run(const BatchArgs &a, int grid, hipStream_t s, int mode, int path);
 };
 
 }  // namespace lbm

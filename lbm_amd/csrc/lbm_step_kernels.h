@@ -223,18 +223,22 @@ void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags
 // batch are small, so the bytes in flight per CU come from occupancy: the
 // D2Q9 FP32 boundary-tile kernel (87 VGPRs, 5 waves unconstrained) runs
 // 10 % faster held to 8 waves despite the spill this costs in its boundary
-// paths.  Interior tiles have their own kernel without boundary code.
-template <int Q, typename ST, bool INT>
+// paths.  PATH 1: tiles whose waves are all interior (no boundary code);
+// 2: any tile, each wave taking the path of its class.
+template <int Q, typename ST, int PATH>
 struct BatchWaves {
   static constexpr int value = 1;
 };
+#ifndef LBM_BATCH_WAVES
+#define LBM_BATCH_WAVES 8
+#endif
 template <>
-struct BatchWaves<9, float, false> {
-  static constexpr int value = 8;
+struct BatchWaves<9, float, 2> {
+  static constexpr int value = LBM_BATCH_WAVES;
 };
 
-template <int Q, typename ST, int MODE, bool NT, bool INT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchWaves<Q, ST, INT>::value))) k_batch_step(const BatchArgs a) {
+template <int Q, typename ST, int MODE, bool NT, int PATH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchWaves<Q, ST, PATH>::value))) k_batch_step(const BatchArgs a) {
   using C = typename Store<ST>::C;
   using V4 = typename Vec4<C>::T;
   // Workgroups go to the 8 XCDs in turn, and each XCD has its own L2.  With
@@ -253,9 +257,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchW
   }
   for (; k < k_end; k += k_step) {
     const int t = a.ids[k];
-    const int4 tl = a.tiles[t];
-    if (!a.active[tl.x]) continue;  // uniform per block
-    const DomainDev &d = a.doms[tl.x];
+    const BatchTile tl = a.tiles[t];
+    if (!a.active[tl.d]) continue;  // uniform per block
+    const DomainDev &d = a.doms[tl.d];
     BatchView v;
     v.src = static_cast<const ST *>(a.src) + d.pop_off;
     v.dst = static_cast<ST *>(a.dst) + d.pop_off;
@@ -273,16 +277,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchW
     v.F[0] = v.F[1] = v.F[2] = 0.0;
     v.snap_in = a.snap_in ? static_cast<V4 *>(a.snap_in) + d.cell_off : nullptr;
     v.snap_out = a.snap_out ? static_cast<V4 *>(a.snap_out) + d.cell_off : nullptr;
-    const int x = tl.y + (int)threadIdx.x % a.tx;
-    const int y = tl.z + (int)threadIdx.x / a.tx;
-    const int z = tl.w & 0x3fffffff;
+    // lane i: cell s + i of the tile's rectangle; lanes past n stay idle
+    // (x = nx fails cell_update's bounds test)
+    int x = d.nx, y = 0;
+    if ((int)threadIdx.x < tl.n) {
+      const int j = tl.s + (int)threadIdx.x;
+      const int r = j / tl.w;
+      x = tl.xs + (j - r * tl.w);
+      y = tl.ys + r;
+    }
+    const int z = tl.z;
     double red[NSUM];
 #pragma unroll
     for (int k = 0; k < NSUM; ++k) red[k] = 0;
-    // tiles with no solid cell within one cell of them skip the flag loads
-    if (INT) cell_update<Q, ST, false, MODE, NT, false, 1>(v, x, y, z, red);
-    else if (tl.w >> 30) cell_update<Q, ST, true, MODE, NT, false>(v, x, y, z, red);
-    else cell_update<Q, ST, false, MODE, NT, false>(v, x, y, z, red);
+    // the path of this wave (uniform per wave): no boundary code (no solid
+    // lane, no face or solid within one cell), the boundary paths with or
+    // without the flag loads, or none (no fluid lane)
+    const int wcls = (tl.cls >> (2 * ((int)threadIdx.x >> 6))) & 3;
+    if (PATH == 1 || wcls == 0) cell_update<Q, ST, false, MODE, NT, false, 1>(v, x, y, z, red);
+    else if (wcls == 2) cell_update<Q, ST, true, MODE, NT, false>(v, x, y, z, red);
+    else if (wcls == 1) cell_update<Q, ST, false, MODE, NT, false>(v, x, y, z, red);
     if (MODE >= 2) {
       block_reduce_store<256>(red, a.partials, t, a.ntiles);
       __syncthreads();  // the LDS of the reduction is reused by the next tile
@@ -290,16 +304,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BatchW
   }
 }
 
-template <int Q, typename ST, bool INT>
+template <int Q, typename ST, int PATH>
 static void launch_batch_kernel(const BatchArgs &a, int grid, hipStream_t s, int mode) {
-  if (mode == 0) hipLaunchKernelGGL((k_batch_step<Q, ST, 0, true, INT>), dim3(grid), dim3(256), 0, s, a);
-  else if (mode == 1) hipLaunchKernelGGL((k_batch_step<Q, ST, 1, false, INT>), dim3(grid), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_batch_step<Q, ST, 2, false, INT>), dim3(grid), dim3(256), 0, s, a);
+  if (mode == 0) hipLaunchKernelGGL((k_batch_step<Q, ST, 0, true, PATH>), dim3(grid), dim3(256), 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL((k_batch_step<Q, ST, 1, false, PATH>), dim3(grid), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_batch_step<Q, ST, 2, false, PATH>), dim3(grid), dim3(256), 0, s, a);
 }
 template <int Q, typename ST>
-void BatchStepL<Q, ST>::run(const BatchArgs &a, int grid, hipStream_t s, int mode, bool interior) {
-  if (interior) launch_batch_kernel<Q, ST, true>(a, grid, s, mode);
-  else launch_batch_kernel<Q, ST, false>(a, grid, s, mode);
+void BatchStepL<Q, ST>::run(const BatchArgs &a, int grid, hipStream_t s, int mode, int path) {
+  if (path == 1) launch_batch_kernel<Q, ST, 1>(a, grid, s, mode);
+  else launch_batch_kernel<Q, ST, 2>(a, grid, s, mode);
 }
 
 }  // namespace lbm

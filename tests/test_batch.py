@@ -41,13 +41,22 @@ def random_init(doms, seed=1):
     return rhos, us
 
 
-@pytest.mark.parametrize("storage,tile", [("f32", 64), ("f64", 64), ("f16", 64), ("f32", 32), ("f32", 128)])
-def test_batch_equals_single_domain_over_graph_replays(storage, tile):
+@pytest.mark.parametrize("storage,tile,blocks,one", [("f32", 64, 0, 1), ("f64", 64, 0, 1), ("f16", 64, 0, 1),
+                                                     ("f32", 32, 0, 1), ("f32", 128, 0, 1), ("f32", 0, 0, 1),
+                                                     ("f64", 0, 0, 1), ("f16", 0, 0, 1), ("f32", 0, 24, 1),
+                                                     ("f32", 16, 0, 1), ("f32", 32, 0, 0), ("f64", 64, 0, 0),
+                                                     ("f32", 0, 0, 0), ("f32", 64, 24, 0)])
+def test_batch_equals_single_domain_over_graph_replays(storage, tile, blocks, one):
+    """tile 0: row runs (a NEEM face column is a tile of its own, so its
+    cells gather their inward neighbours instead of taking them from the
+    next lane); blocks > 0: a persistent grid; one 0: interior tiles and
+    the others in two launches."""
     doms = small_domains()
     rhos, us = random_init(doms)
-    print(f"batch vs single {storage} tile {tile}", flush=True)
+    print(f"batch vs single {storage} tile {tile} blocks {blocks} one launch {one}", flush=True)
     b = capi.Batch(doms, storage=storage)
-    b.set_tiling(tile, 0)
+    b.set_tiling(tile, blocks)
+    b.set_launch(one)
     b.init(np.concatenate(rhos), np.concatenate(us))
     singles = []
     for d, r, u in zip(doms, rhos, us):
@@ -86,7 +95,9 @@ def interior_domains(dim):
 
 
 @pytest.mark.parametrize("dim,storage,tile", [(2, "f32", 32), (2, "f32", 64), (2, "f64", 32), (2, "f16", 64),
-                                              (2, "bf16", 32), (3, "f32", 32), (3, "f32", 64)])
+                                              (2, "bf16", 32), (3, "f32", 32), (3, "f32", 64), (2, "f32", 0),
+                                              (2, "f64", 0), (2, "bf16", 0), (3, "f32", 0), (3, "f64", 0),
+                                              (2, "f32", 16), (3, "f32", 16)])
 def test_batch_interior_tiles_equal_single_domain(dim, storage, tile):
     """Interior tiles (their own kernel) and boundary tiles together give
     results bit-identical to the single-domain step, in every storage and in
@@ -98,22 +109,52 @@ def test_batch_interior_tiles_equal_single_domain(dim, storage, tile):
         n = int(np.prod(d["shape"]))
         rhos.append(1 + 0.01 * rng.standard_normal(n))
         us.append(0.02 * rng.standard_normal((n, dim)))
-    print(f"interior tiles {dim}D {storage} tile {tile}", flush=True)
-    b = capi.Batch(doms, storage=storage)
+    for one in (1, 0):
+        print(f"interior tiles {dim}D {storage} tile {tile} one launch {one}", flush=True)
+        b = capi.Batch(doms, storage=storage)
+        b.set_tiling(tile, 0)
+        b.set_launch(one)
+        b.init(np.concatenate(rhos), np.concatenate(us))
+        singles = []
+        for d, r, u in zip(doms, rhos, us):
+            lat = capi.Lattice(d["shape"], storage=storage, omega=d["omega"], faces=d["faces"] or None,
+                               solid=d["solid"])
+            lat.init(r, u)
+            singles.append(lat)
+        for replay in range(2):
+            b.run(8, 0.0, 8)
+            for k, lat in enumerate(singles):
+                lat.step(8)
+                assert np.array_equal(b.populations(k), lat.populations()), (one, replay, k)
+        for lat in singles:
+            lat.close()
+        b.close()
+
+
+@pytest.mark.parametrize("tile,one", [(16, 1), (32, 1), (64, 1), (16, 0), (32, 0)])
+def test_batch_waves_of_solid_cells(tile, one):
+    """A tall solid block on the domain's corner: tiles whose waves hold
+    only solid cells next to waves with fluid ones (such a wave does no
+    update; a wave with solid and fluid lanes takes the flag path), equal to
+    the single-domain step."""
+    doms = []
+    for k, (nx, ny) in enumerate([(96, 40), (80, 48)]):
+        solid = np.zeros((ny, nx), dtype=np.uint8)
+        solid[:13 + 4 * k, :40] = 1
+        doms.append({"shape": (nx, ny), "omega": 1.3, "solid": solid.reshape(-1),
+                     "faces": W.channel_faces(0.03)})
+    rhos, us = random_init(doms, seed=4)
+    print(f"solid waves tile {tile} one launch {one}", flush=True)
+    b = capi.Batch(doms, storage="f32")
     b.set_tiling(tile, 0)
+    b.set_launch(one)
     b.init(np.concatenate(rhos), np.concatenate(us))
-    singles = []
-    for d, r, u in zip(doms, rhos, us):
-        lat = capi.Lattice(d["shape"], storage=storage, omega=d["omega"], faces=d["faces"] or None,
-                           solid=d["solid"])
-        lat.init(r, u)
-        singles.append(lat)
-    for replay in range(2):
-        b.run(8, 0.0, 8)
-        for k, lat in enumerate(singles):
-            lat.step(8)
-            assert np.array_equal(b.populations(k), lat.populations()), (replay, k)
-    for lat in singles:
+    b.run(6, 0.0, 12)
+    for k, d in enumerate(doms):
+        lat = capi.Lattice(d["shape"], storage="f32", omega=d["omega"], faces=d["faces"], solid=d["solid"])
+        lat.init(rhos[k], us[k])
+        lat.step(12)
+        assert np.array_equal(b.populations(k), lat.populations()), k
         lat.close()
     b.close()
 
@@ -170,8 +211,8 @@ def test_autotuner_keeps_results():
     for tune in (False, True):
         b = capi.Batch(doms, storage="f32")
         if tune:
-            tx, blocks, ms = b.tune(4)
-            assert tx in (32, 64, 128, 256) and blocks >= 0
+            tx, blocks, one, ms = b.tune(4)
+            assert tx in (0, 16, 32, 64, 128, 256) and blocks >= 0 and one in (0, 1)
         b.init(np.concatenate(rhos), np.concatenate(us))
         b.run(4, 0.0, 12)
         out.append([b.populations(k) for k in range(len(doms))])
