@@ -265,7 +265,7 @@ def warmstart_eval(args):
         losses = [float("nan")]
     else:
         doms, x = WS.make_set(args.ntrain, seed=11)
-        steps, res, wall_gen, _, fields = WS.solve(doms, 1e-5, args.max_steps)
+        steps, res, wall_gen, _, fields = WS.solve(doms, 1e-5, args.max_steps, log=log)
         ok = steps > 0
         log(f"# training set: {int(ok.sum())}/{len(doms)} channels converged to 1e-5 in {wall_gen:.1f} s "
             f"(steps {int(steps[ok].min())}..{int(steps[ok].max())})")

@@ -38,14 +38,29 @@ def _init(b, doms, fields=None):
     b.init(np.concatenate(rho), np.concatenate(u))
 
 
-def solve(doms, tol, max_steps, storage="f32", fields=None, interval=100, tiling=None):
-    """Run a batch to tol; returns (steps[n], res[n], wall seconds, [(rho, u)])."""
+def solve(doms, tol, max_steps, storage="f32", fields=None, interval=100, tiling=None, log=None, chunk=20000):
+    """Run a batch to tol; returns (steps[n], res[n], wall seconds, device
+    seconds, [(rho, u)]).  With log, the run goes in pieces of `chunk` steps
+    and each piece reports how many domains have retired."""
     b = capi.Batch(doms, storage=storage)
     if tiling:
         b.set_tiling(*tiling)
     _init(b, doms, fields)
     t0 = time.perf_counter()
-    steps, res, _, ms = b.run(interval, tol, max_steps)
+    if log is None:
+        steps, res, _, ms = b.run(interval, tol, max_steps)
+    else:
+        done, ms = 0, 0.0
+        piece = max(interval, chunk - chunk % interval)
+        while done < max_steps:
+            n = min(piece, max_steps - done)
+            steps, res, _, dms = b.run(interval, tol, n)
+            ms += dms
+            done += n
+            retired = int((steps != -1).sum())
+            log(f"# {done} steps: {retired}/{len(doms)} domains retired, {time.perf_counter() - t0:.1f} s")
+            if retired == len(doms):
+                break
     wall = time.perf_counter() - t0
     out = [b.macros(k) for k in range(len(doms))]
     b.close()
