@@ -47,6 +47,9 @@ struct KArgs {
   void *snap_in;
   void *snap_out;
   double *partials;
+  // two-launch plain steps: bit i of lanes[w] marks lane i of wave w of the
+  // step grid as an interior cell (no face, wrap or solid within one cell)
+  const unsigned long long *lanes;
 };
 
 template <typename C>

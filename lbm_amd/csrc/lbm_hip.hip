@@ -250,6 +250,11 @@ struct lbm_handle {
   bool w64 = false;              // 64-bit population offsets (plane * esize >= 2^32)
   bool lds = false;              // LDS-staged x halo (lbm_set_halo)
   bool forcing = false;          // lbm_set_pressure_forcing
+  // steps of a domain with faces or solids choose a path per wave: the
+  // waves of interior cells only run without the boundary code;
+  // LBM_STEP_WAVE_PATHS=0 keeps the general path for every wave
+  unsigned long long *lanes = nullptr;
+  bool wave_paths = true;
   double p_in = 1.0, p_out = 1.0;
   double *fmac = nullptr;        // (rho, u_x, u_y) of the forced cells before the correction
   long long steps = 0;
@@ -414,6 +419,7 @@ static KArgs make_args(lbm_handle *h) {
   a.face = h->dface;
   a.omega = h->omega;
   for (int i = 0; i < 3; ++i) a.F[i] = h->F[i];
+  a.lanes = h->wave_paths ? h->lanes : nullptr;
   return a;
 }
 
@@ -504,13 +510,41 @@ static void launch_links(const uint8_t *flags, uint32_t *links, int nx, int ny, 
                      per[0], per[1], per[2]);
 }
 
-// (Re)build the solid-link masks; they depend on the periodic axes, so
-// lbm_set_bc calls this again.
+// lanes[w], one word per wave of the step grid (the step kernels' block
+// shape): bit i set when lane i's cell is interior -- inside the domain, one
+// cell or more from every face (no wrap) and without a link to a solid or
+// solid itself (links == 0).  The step updates a wave whose every bit is
+// set without the boundary code.
+__global__ void __launch_bounds__(256) k_lanes(const uint32_t *links, int nx, int ny, int nz, int px, int dim,
+                                               unsigned long long *lanes) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  bool in = x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && (dim == 2 || (z >= 1 && z <= nz - 2));
+  if (in && links) in = links[x + (long long)px * (y + (long long)ny * z)] == 0u;
+  const unsigned long long m = __ballot(in);
+  const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+  const long long wave =
+      ((long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * (blockDim.x * blockDim.y / 64) +
+      (t >> 6);
+  if ((t & 63) == 0) lanes[wave] = m;
+}
+
+// (Re)build the solid-link masks and the interior lanes; they depend on the
+// periodic axes, so lbm_set_bc calls this again.
 static int build_links(lbm_handle *h) {
-  if (!h->has_flags) return LBM_OK;
-  if (h->Q == 9) launch_links<9>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
-  else launch_links<19>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
-  HIPCHK(h, hipGetLastError());
+  if (h->has_flags) {
+    if (h->Q == 9) launch_links<9>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
+    else launch_links<19>(h->flags, h->links, h->nx, h->ny, h->nz, h->px, h->per, h->stream);
+    HIPCHK(h, hipGetLastError());
+  }
+  if (h->lanes) {
+    dim3 g, b;
+    launch_dims(h, g, b);
+    hipLaunchKernelGGL(k_lanes, g, b, 0, h->stream, (const uint32_t *)(h->has_flags ? h->links : nullptr), h->nx, h->ny,
+                       h->nz, h->px, h->dim, h->lanes);
+    HIPCHK(h, hipGetLastError());
+  }
   return LBM_OK;
 }
 
@@ -594,9 +628,14 @@ int lbm_create(const lbm_grid_desc *grid, const lbm_params *p, lbm_handle **out)
       return bail(LBM_ENOMEM);
     }
     h->has_flags = true;
-    rc = build_links(h);
-    if (rc) return bail(rc);
   }
+  if (const char *e = getenv("LBM_STEP_WAVE_PATHS")) h->wave_paths = atoi(e) != 0;
+  if (hipMalloc(&h->lanes, nblocks_of(h) * 4 * sizeof(unsigned long long)) != hipSuccess) {
+    fail(nullptr, LBM_ENOMEM, "lbm_create: hipMalloc of the interior-lane masks failed");
+    return bail(LBM_ENOMEM);
+  }
+  rc = build_links(h);
+  if (rc) return bail(rc);
   rc = ensure_partials(h);
   if (rc) return bail(rc);
   *out = h;
@@ -1020,8 +1059,21 @@ int lbm_set_launch(lbm_handle *h, int32_t block_x, int32_t block_y, int32_t nont
   int bx = block_x > 0 ? block_x : h->bx;
   int by = block_y > 0 ? block_y : h->by;
   if (bx % 64 != 0 || bx * by != 256) return fail(h, LBM_EINVAL, "lbm_set_launch: block_x must be a multiple of 64 and block_x*block_y == 256");
-  h->bx = bx;
-  h->by = by;
+  if (bx != h->bx || by != h->by) {
+    // the interior-lane masks follow the waves of the step grid: new masks
+    // for the new shape first, so that a failure leaves the handle as it was
+    const long long nb = (long long)((h->nx + bx - 1) / bx) * ((h->ny + by - 1) / by) * h->nz;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    unsigned long long *lanes = nullptr;
+    if (hipMalloc(&lanes, nb * 4 * sizeof(unsigned long long)) != hipSuccess)
+      return fail(h, LBM_ENOMEM, "lbm_set_launch: hipMalloc of the interior-lane masks failed");
+    if (h->lanes) (void)hipFree(h->lanes);
+    h->lanes = lanes;
+    h->bx = bx;
+    h->by = by;
+    const int rc = build_links(h);
+    if (rc) return rc;
+  }
   if (nontemporal >= 0) h->nt = nontemporal;
   return ensure_partials(h);
 }
@@ -1090,6 +1142,7 @@ int lbm_destroy(lbm_handle *h) {
   if (h->flags) hipFree(h->flags);
   if (h->fmac) hipFree(h->fmac);
   if (h->links) hipFree(h->links);
+  if (h->lanes) hipFree(h->lanes);
   for (int b = 0; b < 2; ++b)
     if (h->snap[b]) hipFree(h->snap[b]);
   if (h->partials) hipFree(h->partials);

@@ -17,6 +17,10 @@ template <>
 struct StepWaves<19, float, 0> {
   static constexpr int value = 4;
 };
+template <>
+struct StepWaves<19, float, 3> {
+  static constexpr int value = 4;
+};
 // 16-bit storage (FP32 arithmetic), general kernel: 104 VGPRs unconstrained
 // (4 waves); held to 6 waves it spills 92-96 bytes per lane in its boundary
 // paths and still runs the 512x128x128 sphere channel 21-26 % faster (FP16
@@ -30,8 +34,23 @@ template <>
 struct StepWaves<19, __hip_bfloat16, 0> {
   static constexpr int value = 6;
 };
+template <>
+struct StepWaves<19, __half, 3> {
+  static constexpr int value = 6;
+};
+template <>
+struct StepWaves<19, __hip_bfloat16, 3> {
+  static constexpr int value = 6;
+};
 
 
+// TOPO 3 (domains with faces or solids, a.lanes set): a path per wave -- a
+// wave whose cells are all interior (every bit of a.lanes[wave]: no face,
+// wrap or solid within one cell) runs without the boundary code and the
+// flag loads (TOPO 1), the others run the general path (TOPO 0).  Splitting
+// the same classes into two launches was measured slower on the sphere
+// channel (the boundary launch's blocks hold one working wave in four, and
+// the halo lines the two classes share are fetched twice).
 template <int Q, typename ST, bool FLAGS, int MODE, bool NT, bool W64, int TOPO>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWaves<Q, ST, TOPO>::value))) k_step(const KArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,7 +59,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(StepWa
   double red[NSUM];
 #pragma unroll
   for (int k = 0; k < NSUM; ++k) red[k] = 0;
-  cell_update<Q, ST, FLAGS, MODE, NT, W64, TOPO>(a, x, y, z, red);
+  if constexpr (TOPO == 3) {
+    const int t = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    const long long wave =
+        ((long long)(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * (blockDim.x * blockDim.y / 64) +
+        (t >> 6);
+    if (a.lanes[wave] == ~0ull) cell_update<Q, ST, false, MODE, NT, W64, 1>(a, x, y, z, red);
+    else cell_update<Q, ST, FLAGS, MODE, NT, W64, 0>(a, x, y, z, red);
+  } else {
+    cell_update<Q, ST, FLAGS, MODE, NT, W64, TOPO>(a, x, y, z, red);
+  }
   if (MODE >= 2) {
     const long long nb = (long long)gridDim.x * gridDim.y * gridDim.z;
     const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
@@ -209,6 +237,16 @@ void StepL<Q, ST>::run(const KArgs &a, dim3 g, dim3 b, hipStream_t s, bool flags
       else launch_step_kernel<Q, ST, false, false, 2>(a, g, b, s, mode, nt);
       return;
     }
+  }
+  if (a.lanes && !periodic_box) {
+    if (w64) {
+      if (flags) launch_step_kernel<Q, ST, true, true, 3>(a, g, b, s, mode, nt);
+      else launch_step_kernel<Q, ST, false, true, 3>(a, g, b, s, mode, nt);
+    } else {
+      if (flags) launch_step_kernel<Q, ST, true, false, 3>(a, g, b, s, mode, nt);
+      else launch_step_kernel<Q, ST, false, false, 3>(a, g, b, s, mode, nt);
+    }
+    return;
   }
   if (w64) {
     if (flags) launch_step_kernel<Q, ST, true, true, 0>(a, g, b, s, mode, nt);
