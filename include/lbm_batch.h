@@ -41,7 +41,8 @@ typedef struct {
 int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *doms, int32_t ndom, lbm_batch **out);
 
 /* Autotuner: times every candidate tiling (2D tiles of 256 cells and width
- * 16/32/64/128/256, or tile_x 0: row runs, 256 consecutive cells of a
+ * 16/32/64/128/256; tile_x 1: banded, 16x16 tiles along both x faces and
+ * 64x4 tiles between; tile_x 0: row runs, 256 consecutive cells of a
  * rectangle of cells of one class -- interior, boundary, next to a solid;
  * each wave of 64 cells takes the boundary paths only when one of its cells
  * needs them; persistent grids of 2..16 blocks per CU or one block per tile)

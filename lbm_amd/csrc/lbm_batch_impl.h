@@ -178,37 +178,60 @@ static void domain_tiles(const DomainDev &D, const std::vector<uint8_t> &cc, int
   const int nx = D.nx, ny = D.ny, nz = D.nz;
   constexpr uint8_t SOLID = CELL_SOLID;
   auto at = [&](int x, int y, int z) { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z); };
+  // one 2D tile: the per-wave paths of its lanes (lane i: cell
+  // (x0 + i % w, y0 + i / w)), left out when it holds no fluid cell
+  auto tile = [&](int x0, int w, int y0, int z) {
+    int mask = 0;
+    bool fluid = false;
+    for (int wv = 0; wv < 4; ++wv) {
+      // the path without boundary code never sees a solid lane: it would
+      // update the cell as fluid and, on a domain edge, pull from outside
+      // the domain
+      int wc = 0;
+      bool wfluid = false, wsolid = false;
+      for (int i = 64 * wv; i < 64 * wv + 64; ++i) {
+        const int x = x0 + i % w, y = y0 + i / w;
+        if (x >= nx || y >= ny) continue;
+        const uint8_t k = cc[at(x, y, z)];
+        if (k == SOLID) {
+          wsolid = true;
+          continue;
+        }
+        wfluid = true;
+        wc = std::max(wc, (int)k);
+      }
+      if (!wfluid) wc = 3;      // nothing to update: the wave skips the cell update
+      else if (wsolid) wc = 2;  // flag loads deactivate the solid lanes
+      fluid = fluid || wfluid;
+      mask |= wc << (2 * wv);
+    }
+    if (fluid) out.push_back(BatchTile{d, x0, w, y0, 0, 256, z, mask});
+  };
+  if (tx == 1) {
+    // banded: 16x16 tiles over the first 16 columns and from the last
+    // 64-wide tile on, 64x4 tiles between, so that a wave holding a face
+    // cell of an x face holds 16 columns instead of 64 (16 % of the fluid
+    // cells of the 2048 step channels in waves with a boundary cell
+    // instead of 33 %); tiles go by 16-row bands, column by column
+    std::vector<std::pair<int, int>> cols;
+    int x = std::min(16, nx);
+    cols.push_back({0, 16});
+    while (x + 64 <= nx - 16) {
+      cols.push_back({x, 64});
+      x += 64;
+    }
+    for (; x < nx; x += 16) cols.push_back({x, 16});
+    for (int z = 0; z < nz; ++z)
+      for (int b0 = 0; b0 < ny; b0 += 16)
+        for (const auto &c : cols)
+          for (int y0 = b0; y0 < std::min(ny, b0 + 16); y0 += 256 / c.second) tile(c.first, c.second, y0, z);
+    return;
+  }
   if (tx > 0) {
     const int ty = 256 / tx;
     for (int z = 0; z < nz; ++z)
       for (int y0 = 0; y0 < ny; y0 += ty)
-        for (int x0 = 0; x0 < nx; x0 += tx) {
-          int mask = 0;
-          bool fluid = false;
-          for (int w = 0; w < 4; ++w) {
-            // the path without boundary code never sees a solid lane: it
-            // would update the cell as fluid and, on a domain edge, pull
-            // from outside the domain
-            int wc = 0;
-            bool wfluid = false, wsolid = false;
-            for (int i = 64 * w; i < 64 * w + 64; ++i) {
-              const int x = x0 + i % tx, y = y0 + i / tx;
-              if (x >= nx || y >= ny) continue;
-              const uint8_t k = cc[at(x, y, z)];
-              if (k == SOLID) {
-                wsolid = true;
-                continue;
-              }
-              wfluid = true;
-              wc = std::max(wc, (int)k);
-            }
-            if (!wfluid) wc = 3;            // nothing to update: the wave skips the cell update
-            else if (wsolid) wc = 2;        // flag loads deactivate the solid lanes
-            fluid = fluid || wfluid;
-            mask |= wc << (2 * w);
-          }
-          if (fluid) out.push_back(BatchTile{d, x0, tx, y0, 0, 256, z, mask});
-        }
+        for (int x0 = 0; x0 < nx; x0 += tx) tile(x0, tx, y0, z);
     return;
   }
   struct Run {
@@ -510,9 +533,9 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
 }
 
 int lbm_batch_set_tiling(lbm_batch *b, int32_t tile_x, int32_t blocks) {
-  if (!b || !(tile_x == 0 || tile_x == 16 || tile_x == 32 || tile_x == 64 || tile_x == 128 || tile_x == 256) ||
-      blocks < 0)
-    return bfail(b, LBM_EINVAL, "lbm_batch_set_tiling: tile_x in {0,16,32,64,128,256}, blocks >= 0");
+  if (!b || !(tile_x == 0 || tile_x == 1 || tile_x == 16 || tile_x == 32 || tile_x == 64 || tile_x == 128 ||
+              tile_x == 256) || blocks < 0)
+    return bfail(b, LBM_EINVAL, "lbm_batch_set_tiling: tile_x in {0,1,16,32,64,128,256}, blocks >= 0");
   b->blocks = blocks;
   return batch_build_tiles(b, tile_x);
 }
@@ -570,7 +593,7 @@ int lbm_batch_tune(lbm_batch *b, int32_t steps_per_trial, char *choice, int32_t 
     ms = t / nst;
     return LBM_OK;
   };
-  const int txs[6] = {0, 16, 32, 64, 128, 256};
+  const int txs[7] = {0, 1, 16, 32, 64, 128, 256};
   const int per_cu[4] = {0, 2, 4, 16};
   std::vector<Cand> cands;
   for (int tx : txs)

@@ -45,7 +45,8 @@ def random_init(doms, seed=1):
                                                      ("f32", 32, 0, 1), ("f32", 128, 0, 1), ("f32", 0, 0, 1),
                                                      ("f64", 0, 0, 1), ("f16", 0, 0, 1), ("f32", 0, 24, 1),
                                                      ("f32", 16, 0, 1), ("f32", 32, 0, 0), ("f64", 64, 0, 0),
-                                                     ("f32", 0, 0, 0), ("f32", 64, 24, 0)])
+                                                     ("f32", 0, 0, 0), ("f32", 64, 24, 0), ("f32", 1, 0, 1),
+                                                     ("f16", 1, 0, 1), ("f32", 1, 0, 0)])
 def test_batch_equals_single_domain_over_graph_replays(storage, tile, blocks, one):
     """tile 0: row runs (a NEEM face column is a tile of its own, so its
     cells gather their inward neighbours instead of taking them from the
@@ -97,7 +98,7 @@ def interior_domains(dim):
 @pytest.mark.parametrize("dim,storage,tile", [(2, "f32", 32), (2, "f32", 64), (2, "f64", 32), (2, "f16", 64),
                                               (2, "bf16", 32), (3, "f32", 32), (3, "f32", 64), (2, "f32", 0),
                                               (2, "f64", 0), (2, "bf16", 0), (3, "f32", 0), (3, "f64", 0),
-                                              (2, "f32", 16), (3, "f32", 16)])
+                                              (2, "f32", 16), (3, "f32", 16), (2, "f32", 1), (3, "f32", 1)])
 def test_batch_interior_tiles_equal_single_domain(dim, storage, tile):
     """Interior tiles (their own kernel) and boundary tiles together give
     results bit-identical to the single-domain step, in every storage and in
@@ -131,7 +132,7 @@ def test_batch_interior_tiles_equal_single_domain(dim, storage, tile):
         b.close()
 
 
-@pytest.mark.parametrize("tile,one", [(16, 1), (32, 1), (64, 1), (16, 0), (32, 0)])
+@pytest.mark.parametrize("tile,one", [(16, 1), (32, 1), (64, 1), (16, 0), (32, 0), (1, 1)])
 def test_batch_waves_of_solid_cells(tile, one):
     """A tall solid block on the domain's corner: tiles whose waves hold
     only solid cells next to waves with fluid ones (such a wave does no
@@ -212,7 +213,7 @@ def test_autotuner_keeps_results():
         b = capi.Batch(doms, storage="f32")
         if tune:
             tx, blocks, one, ms = b.tune(4)
-            assert tx in (0, 16, 32, 64, 128, 256) and blocks >= 0 and one in (0, 1)
+            assert tx in (0, 1, 16, 32, 64, 128, 256) and blocks >= 0 and one in (0, 1)
         b.init(np.concatenate(rhos), np.concatenate(us))
         b.run(4, 0.0, 12)
         out.append([b.populations(k) for k in range(len(doms))])
