@@ -527,17 +527,17 @@ def main():
 
 
 def time_to_residual():
-    """The metric's second half, bounded to well under a minute of GPU time:
+    """The metric's second half, bounded to about a minute of GPU time:
     the 512x128x128 sphere channel to the relative residual 1e-6 in FP32
     (capped at 80,000 steps), and the learned warm start against the plain
-    FP32 start on 4 unseen 256x128 obstacle channels with a network trained
-    in this run on 256 channels (+ mirrored, 20 epochs) by the project's own
-    kernels.  Skipped under rocprofv3 (the warm start's batched solves do
-    not run under it)."""
+    FP32 start on 8 unseen 256x128 obstacle channels with a network trained
+    in this run on 1024 channels (+ mirrored, 30 epochs) by the project's
+    own kernels.  Skipped under rocprofv3 (the warm start's batched solves
+    do not run under it)."""
     t0 = time.perf_counter()
     sph = sphere_solve("f32", 80000)
     sph.pop("u")
-    ws = warmstart_eval(argparse.Namespace(load_model=None, ntrain=256, ntest=4, epochs=20, lr=5e-4, mirror=1,
+    ws = warmstart_eval(argparse.Namespace(load_model=None, ntrain=1024, ntest=8, epochs=30, lr=5e-4, mirror=1,
                                            max_steps=400000, tol16=1e-3, save_model=None))
     keep = ("cold_f32", "warm_f32", "converged_cases", "prediction_u_rel_l2_median", "predict_s_per_case", "training")
     return {"sphere_512x128x128_f32": sph,
