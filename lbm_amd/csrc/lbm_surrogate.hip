@@ -457,6 +457,8 @@ __global__ void __launch_bounds__(256) k_x_prep(const u16 *A, int ca, const u16 
   }
 }
 
+constexpr int WG_DEPTH = 4;  // K steps of k_wgrad_t in flight per wave
+
 struct WgradTArgs {
   const u16 *gT;  // [co][P]
   const u16 *xT;  // [ci][nb][h + 2][w + 8]
@@ -488,8 +490,8 @@ __device__ __forceinline__ bf16x8 shifted8(const u16x8 lo, const u16x8 hi, int s
 // the three taps of the row (three 32x32 accumulators).  A K step is two
 // segments (lane group g takes segment 2k + g; a segment never crosses an
 // image row since w % 8 == 0): the gradient operand is one 16-byte load,
-// the input row two aligned loads serving the three dx shifts; the next
-// step's loads are issued before this step's products.  The chunk sums in
+// the input row two aligned loads serving the three dx shifts; the loads
+// run WG_DEPTH steps ahead of the products.  The chunk sums in
 // pixel order, so the gradient is the same bit for bit however the loads
 // are scheduled.
 __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
@@ -549,18 +551,24 @@ __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
       }
     }
   };
-  u16x8 av, lo, hi;
-  load(av, lo, hi);
-  for (long long sk = sbeg; sk < send; sk += 2) {  // wave-uniform
-    u16x8 av1 = zero, lo1 = zero, hi1 = zero;
-    if (sk + 2 < send) load(av1, lo1, hi1);
-    const bf16x8 A = __builtin_bit_cast(bf16x8, av);
+  // a ring of WG_DEPTH steps in flight: step k's operands are loaded while
+  // the products of steps k - WG_DEPTH + 1 .. k - 1 run (a wave's chain of
+  // loads and products is the whole cost of this kernel at two waves per
+  // SIMD); the steps still run in order
+  u16x8 av[WG_DEPTH], lo[WG_DEPTH], hi[WG_DEPTH];
 #pragma unroll
-    for (int dx = 0; dx < 3; ++dx)
-      acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, shifted8(lo, hi, dx), acc[dx], 0, 0, 0);
-    av = av1;
-    lo = lo1;
-    hi = hi1;
+  for (int i = 0; i < WG_DEPTH; ++i) load(av[i], lo[i], hi[i]);
+  for (long long sk = sbeg; sk < send; sk += 2 * WG_DEPTH) {
+#pragma unroll
+    for (int i = 0; i < WG_DEPTH; ++i) {
+      if (sk + 2 * i < send) {  // wave-uniform
+        const bf16x8 A = __builtin_bit_cast(bf16x8, av[i]);
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx)
+          acc[dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, shifted8(lo[i], hi[i], dx), acc[dx], 0, 0, 0);
+        load(av[i], lo[i], hi[i]);  // the step WG_DEPTH ahead (zeros past the chunk)
+      }
+    }
   }
   if (!cok) return;
   float *dst = a.part + (long long)chunk * a.co * 9 * a.ci;
