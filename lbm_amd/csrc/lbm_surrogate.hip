@@ -361,16 +361,19 @@ __global__ void __launch_bounds__(256) k_wgrad(const WgradArgs a) {
 // ---------------------------------------------------------------------------
 // Weight gradient on channel-major operands.  The pixel sum is the GEMM's K
 // axis, and NHWC keeps pixels strided; two staging kernels write the layer's
-// output gradient as gT[co][p] and its input, zero padded, as
-// xT[ci][b][h + 2][w + 8] (one pixel of halo on each side, the rows padded
-// to a multiple of 8 elements), both through an LDS tile transpose, so the
-// MFMA operands of k_wgrad_t are 16-byte loads of 8 consecutive pixels.
-// Used where the width at the layer's level is a multiple of 8.
+// output gradient as gT[p / 8][co][8] and its input, zero padded, as
+// xT[b][h + 2][(w + 8) / 8][ci][8] (one pixel of halo on each side, the rows
+// padded to a multiple of 8 elements), both through an LDS tile transpose:
+// groups of 8 consecutive pixels of one channel, the channels of a group
+// next to each other, so the MFMA operands of k_wgrad_t are 16-byte loads of
+// 8 consecutive pixels and the 32 lanes of a channel tile read 512
+// contiguous bytes.  Used where the width at the layer's level is a
+// multiple of 8.
 constexpr int TP = 64;  // pixels per transpose tile
 
 // From the FP32 gradient g [p][co] of a layer's output (relu mask from its
 // bf16 activation y), or from a bf16 gradient gin: gb = bf16 values [p][co]
-// (not written when gin is given), gT [co][p], and per-tile bias sums
+// (not written when gin is given), gT [p / 8][co][8], and per-tile bias sums
 // bpart[tile][co] of the bf16 values (8-pixel groups, then the groups in
 // order).  Block: 64 pixels x 32 channels; grid (ceil(P / 64), co / 32).
 __global__ void __launch_bounds__(256) k_gy_prep(const float *g, const u16 *y, int relu, const u16 *gin, u16 *gb,
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(256) k_gy_prep(const float *g, const u16 *y, i
   u16x8 w;
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] = tile[ch][8 * j + i];
-  if (pp < P) *reinterpret_cast<u16x8 *>(gT + (long long)(c0 + ch) * P + pp) = w;
+  if (pp < P) *reinterpret_cast<u16x8 *>(gT + ((pp >> 3) * co + c0 + ch) * 8) = w;
   float sum = 0.f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) sum += bf2f(w[i]);  // zero beyond P
@@ -423,11 +426,11 @@ __global__ void __launch_bounds__(256) k_gy_prep(const float *g, const u16 *y, i
   }
 }
 
-// xT[c][b][yp][xp] = input channel c of pixel (b, yp - 1, xp - 1), zero
-// outside the image; channels < ca come from the half-resolution tensor A
-// (nearest upsampling), the rest from S.  Block: 64 padded columns x 32
-// channels of one padded row; grid (ceil((w + 8) / 64), nb (h + 2),
-// ceil(ci / 32)).  Requires ca, cs multiples of 8.
+// xT[b][yp][xp / 8][c][xp % 8] = input channel c of pixel (b, yp - 1,
+// xp - 1), zero outside the image; channels < ca come from the
+// half-resolution tensor A (nearest upsampling), the rest from S.  Block: 64
+// padded columns x 32 channels of one padded row; grid (ceil((w + 8) / 64),
+// nb (h + 2), ceil(ci / 32)).  Requires ca, cs multiples of 8.
 __global__ void __launch_bounds__(256) k_x_prep(const u16 *A, int ca, const u16 *S, int cs, int nb, int h, int w,
                                                 u16 *xT) {
   __shared__ u16 tile[32][TP + 8];
@@ -453,15 +456,15 @@ __global__ void __launch_bounds__(256) k_x_prep(const u16 *A, int ca, const u16 
     u16x8 o;
 #pragma unroll
     for (int i = 0; i < 8; ++i) o[i] = tile[ch][8 * j + i];
-    *reinterpret_cast<u16x8 *>(xT + ((long long)(c0 + ch) * nb * hp + row) * wp + xq) = o;
+    *reinterpret_cast<u16x8 *>(xT + (((long long)row * (wp >> 3) + (xq >> 3)) * ci + c0 + ch) * 8) = o;
   }
 }
 
 constexpr int WG_DEPTH = 4;  // K steps of k_wgrad_t in flight per wave
 
 struct WgradTArgs {
-  const u16 *gT;  // [co][P]
-  const u16 *xT;  // [ci][nb][h + 2][w + 8]
+  const u16 *gT;  // [P / 8][co][8]
+  const u16 *xT;  // [nb][h + 2][(w + 8) / 8][ci][8]
   long long P;
   int co, ci, nb, h, w;
   int cseg;     // 8-pixel segments per chunk (even)
@@ -508,8 +511,9 @@ __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
   const int m = lane & 31, g = lane >> 5;
   const int wp = a.w + 8, hp = a.h + 2;
   const bool cok = ci0 + m < a.ci;
-  const u16 *arow = a.gT + (long long)(co0 + m) * a.P;
-  const u16 *xch = a.xT + (long long)(cok ? ci0 + m : 0) * a.nb * hp * wp + (long long)dy * wp;
+  const u16 *arow = a.gT + (long long)(co0 + m) * 8;                       // + segment * co * 8
+  const u16 *xch = a.xT + (long long)(cok ? ci0 + m : 0) * 8 + (long long)dy * (wp >> 3) * a.ci * 8;  // + group * ci * 8
+  const int wg = wp >> 3;
   const long long nseg = a.P >> 3;
   const long long sbeg = (long long)chunk * a.cseg;
   const long long send = sbeg + a.cseg < nseg ? sbeg + a.cseg : nseg;
@@ -534,11 +538,12 @@ __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
     lo = zero;
     hi = zero;
     if (s < send) {
-      av = *reinterpret_cast<const u16x8 *>(arow + (s << 3));
+      av = *reinterpret_cast<const u16x8 *>(arow + s * a.co * 8);
       if (cok) {
-        const u16 *xr = xch + ((long long)b * hp + y) * wp + x;
+        // padded columns x .. x + 15: the groups x / 8 and x / 8 + 1 of the row
+        const u16 *xr = xch + (((long long)b * hp + y) * wg + (x >> 3)) * a.ci * 8;
         lo = *reinterpret_cast<const u16x8 *>(xr);
-        hi = *reinterpret_cast<const u16x8 *>(xr + 8);
+        hi = *reinterpret_cast<const u16x8 *>(xr + (long long)a.ci * 8);
       }
     }
     s += 2;
