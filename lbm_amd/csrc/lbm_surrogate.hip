@@ -54,7 +54,7 @@ struct ConvArgs {
   const u16 *a;  // upsampled input (stored at half resolution), ca channels, may be null
   const u16 *s;  // same-resolution input, cs channels
   int ca, cs;
-  const u16 *w;  // [co][9][ca + cs]
+  const u16 *w;  // [9][(ca + cs) / 16][co][16]: K step k of output channel o at (k * co + o) * 16
   const float *bias;
   int nb, h, wd, co;
   // MODE 0: y = bf16(act(acc + bias)), [p][co]
@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(256) k_conv(const ConvArgs a, const int ksplit
     y = (int)((p / a.wd) % a.h);
     b = (int)(p / ((long long)a.wd * a.h));
   }
-  const u16 *wrow = a.w + (long long)(co0 + m) * 9 * ci + 8 * g;
+  const u16 *wrow = a.w + (long long)(co0 + m) * 16 + 8 * g;
+  const long long wstep = (long long)a.co * 16;  // elements per K step
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(256) k_conv(const ConvArgs a, const int ksplit
   auto load = [&](int k, u16x8 &av, u16x8 &bv) {
     const int tap = k / cpt, c = (k - tap * cpt) << 4;
     const int ys = y + tap / 3 - 1, xs = x + tap % 3 - 1;
-    bv = *reinterpret_cast<const u16x8 *>(wrow + tap * ci + c);
+    bv = *reinterpret_cast<const u16x8 *>(wrow + k * wstep);
     av = zero;
     if (pin && ys >= 0 && ys < a.h && xs >= 0 && xs < a.wd) {
       if (c < a.ca)
@@ -205,7 +206,8 @@ __global__ void __launch_bounds__(256) k_conv_lds(const ConvArgs a) {
   const int y = y0 + wave;
   if (y >= a.h) return;  // after the only barrier
   const int m = lane & 31, g = lane >> 5;
-  const u16 *wrow = a.w + (long long)(co0 + m) * 9 * ci + 8 * g;
+  const u16 *wrow = a.w + (long long)(co0 + m) * 16 + 8 * g;
+  const long long wstep = (long long)a.co * 16;  // elements per K step
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
@@ -214,10 +216,7 @@ __global__ void __launch_bounds__(256) k_conv_lds(const ConvArgs a) {
   for (int k = 0; k < nks; ++k) {
     const int tap = k / cpt, c = (k - tap * cpt) << 4;
     u16x8 bv1 = bv;
-    if (k + 1 < nks) {
-      const int t1 = (k + 1) / cpt, c1 = ((k + 1) - t1 * cpt) << 4;
-      bv1 = *reinterpret_cast<const u16x8 *>(wrow + t1 * ci + c1);
-    }
+    if (k + 1 < nks) bv1 = *reinterpret_cast<const u16x8 *>(wrow + (k + 1) * wstep);
     // pixel (y + dy, x0 + m + dx) sits at LDS row wave + 1 + dy, column m + 1 + dx
     const int pix = (wave + tap / 3) * 34 + m + tap % 3;
     const u16x8 av = *reinterpret_cast<const u16x8 *>(xs + pix * cst + c + 8 * g);
@@ -786,8 +785,11 @@ __global__ void __launch_bounds__(256) k_adam(float *p, const float *g, float *m
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// bf16 working copies of every layer in one launch (blockIdx.y = layer):
-// wb[co][tap][ci] and the dgrad form wd[ci][8 - tap][co]
+// bf16 working copies of every layer in one launch (blockIdx.y = layer),
+// in the convolutions' operand order [tap][in / 16][out][16] (K step
+// tap * (in / 16) + c / 16 of output channel o: 16 consecutive input
+// channels, the output channels of a step next to each other): wb, in =
+// ci, out = co; the dgrad form wd, in = co, out = ci, taps flipped
 constexpr int MAXLAYERS = 32;
 struct PrepLayer {
   const float *wf;
@@ -807,8 +809,8 @@ __global__ void __launch_bounds__(256) k_prep_all(const PrepAll a) {
   const int tap = (int)((i / L.ci) % 9);
   const int o = (int)(i / (9LL * L.ci));
   const u16 v = f2bf(L.wf[i]);
-  L.wb[i] = v;
-  L.wd[((long long)c * 9 + (8 - tap)) * L.co + o] = v;
+  L.wb[(((long long)tap * (L.ci >> 4) + (c >> 4)) * L.co + o) * 16 + (c & 15)] = v;
+  L.wd[(((long long)(8 - tap) * (L.co >> 4) + (o >> 4)) * L.ci + c) * 16 + (o & 15)] = v;
 }
 
 // one batch from the device dataset: features, targets and masks of
