@@ -100,7 +100,7 @@ __device__ __forceinline__ void gather(const PArgs &a, int x, int y, double (&g)
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const long long s = (long long)(x - L::cx[i]) + (long long)a.px * (y - L::cy[i]);
-    g[i] = a.src[i * a.plane + s];
+    g[i] = __builtin_nontemporal_load(&a.src[i * a.plane + s]);  // streamed once: 2.195 -> 2.076 ms at 8192^2
   }
 }
 
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_poisson_step(const PArgs a) {
       const double feq = (i == Q - 1 ? L::w[i] - 1.0 : L::w[i]) * phi;
       double f = (1.0 - om) * g[i] + om * feq;
       if (i < Q - 1) f += a.source * L::wp[i] * phi;
-      a.dst[i * a.plane + cell] = f;
+      __builtin_nontemporal_store(f, &a.dst[i * a.plane + cell]);
     }
   }
   if (CHECK) {
