@@ -1120,6 +1120,7 @@ static unsigned conv_blocks(long long npix, int co) {
 
 // the large levels (at least 65536 pixels, width a multiple of 32, at most
 // LDS_CI_MAX input channels) run k_conv_lds
+// (16384 pixels measured the same per training step, 4096 3 % slower)
 static bool conv_lds(long long npix, int w, int ci) { return npix >= 65536 && w % 32 == 0 && ci <= LDS_CI_MAX; }
 
 static void launch_conv(int mode, const ConvArgs &a, hipStream_t st) {
