@@ -467,8 +467,8 @@ struct WgradTArgs {
   long long P;
   int co, ci, nb, h, w;
   int cseg;     // 8-pixel segments per chunk (even)
-  int nchunks;
-  float *part;  // [chunk][co][9][ci]
+  int nchunks;  // a multiple of 4: the 4 waves of a block take 4 chunks of one tile
+  float *part;  // [chunk / 4][co][9][ci]
 };
 
 // 8 bf16 values starting `sh` elements (0, 1 or 2) into lo:hi
@@ -495,12 +495,14 @@ __device__ __forceinline__ bf16x8 shifted8(const u16x8 lo, const u16x8 hi, int s
 // the input row two aligned loads serving the three dx shifts; the loads
 // run WG_DEPTH steps ahead of the products.  The chunk sums in
 // pixel order, so the gradient is the same bit for bit however the loads
-// are scheduled.
+// are scheduled.  The 4 waves of a block take 4 consecutive chunks of one
+// tile and add their sums through LDS in wave order before one partial per
+// block is written.
 __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
+  __shared__ float red[3][3][16][64];  // waves 1..3: [wave - 1][dx][r][lane]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ntco = a.co >> 5, ntci = (a.ci + 31) >> 5;
-  long long t = (long long)blockIdx.x * 4 + wave;
-  if (t >= (long long)ntco * ntci * 3 * a.nchunks) return;
+  long long t = (long long)blockIdx.x * 4 + wave;  // the grid is exactly the tiles x chunks
   const int chunk = (int)(t % a.nchunks);
   t /= a.nchunks;
   const int dy = (int)(t % 3);
@@ -574,8 +576,20 @@ __global__ void __launch_bounds__(256) k_wgrad_t(const WgradTArgs a) {
       }
     }
   }
-  if (!cok) return;
-  float *dst = a.part + (long long)chunk * a.co * 9 * a.ci;
+  if (wave > 0)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[wave - 1][q][r][lane] = acc[q][r];
+  __syncthreads();
+  if (wave > 0 || !cok) return;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[q][r] += red[w][q][r][lane];
+  float *dst = a.part + (long long)(chunk >> 2) * a.co * 9 * a.ci;
 #pragma unroll
   for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -1054,7 +1068,8 @@ static int alloc_all(lbm_sur *s) {
       for (int nb = 1; nb <= s->B; ++nb) {  // the chunking depends on the batch in use
         const long long npb = npix_at(s, s->t[l.s].level, nb);
         const long long cseg = wgrad_cseg(npb, l.co, l.ca + l.cs);
-        s->part_cap = std::max(s->part_cap, (npb / 8 + cseg - 1) / cseg * n);
+        const long long nch = ((npb / 8 + cseg - 1) / cseg + 3) / 4 * 4;  // chunks, a multiple of 4
+        s->part_cap = std::max(s->part_cap, nch / 4 * n);
       }
       s->gT_cap = std::max(s->gT_cap, np * l.co);
       s->xT_cap = std::max(s->xT_cap, (long long)(l.ca + l.cs) * s->B * (h + 2) * (w + 8));
@@ -1238,12 +1253,12 @@ static int loss_backward(lbm_sur *s, int nb, long long nfluid, bool grads) {
       wa.h = h;
       wa.w = w;
       wa.cseg = wgrad_cseg(np, l.co, ci);
-      wa.nchunks = (int)((np / 8 + wa.cseg - 1) / wa.cseg);
+      wa.nchunks = (int)(((np / 8 + wa.cseg - 1) / wa.cseg + 3) / 4 * 4);
       wa.part = s->part;
       const long long waves = (long long)(l.co / 32) * ((ci + 31) / 32) * 3 * wa.nchunks;
       hipLaunchKernelGGL(k_wgrad_t, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s->wg, wa);
       hipLaunchKernelGGL(k_reduce_wb, dim3(nblk32(nw) + l.co), dim3(256), 0, s->wg, (const float *)s->part, nw,
-                         wa.nchunks, s->gp + l.woff, (const float *)bpart, (long long)ptiles, l.co, s->gp + l.boff);
+                         wa.nchunks / 4, s->gp + l.woff, (const float *)bpart, (long long)ptiles, l.co, s->gp + l.boff);
       SCHK(s, hipEventRecord(s->ev_wg[k], s->wg));
       pending[k] = true;
       last = k;
