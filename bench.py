@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# arithmetic type of the step for each population storage form: FP32 and FP64
+# storage collide in FP64, the 16-bit forms in FP32
+COMPUTE = {"f32": "f64", "f64": "f64", "f16": "f32", "bf16": "f32"}
 
 
 def cpu_baseline(seconds_target=12.0):
@@ -133,7 +136,8 @@ def run_batch(args):
     bpp = capi.BYTES[args.storage]
     achieved = updates * (2 * 9 * bpp + 1) / (ms * 1e-3) / 1e9
     line = {"metric": "batched D2Q9 step channels: wall time to residual 1e-6 (all domains)",
-            "value": round(ms / 1e3, 3), "unit": "s", "higher_is_better": False, "dtype": args.storage,
+            "value": round(ms / 1e3, 3), "unit": "s", "higher_is_better": False, "dtype": COMPUTE[args.storage],
+            "storage": args.storage,
             "config": {"workload": f"{args.ndom} backward-facing-step channels L 64..512, H 32..128, step 0.3-0.6 H, "
                                    f"tau 0.55..1.0, inlet u=0.03, seed 2024", "cells": int(cells.sum()),
                        "fluid_cells": int(fluid.sum())},
@@ -318,7 +322,7 @@ def warmstart_eval(args):
     e_pred = [WS.rel_l2(pred[i][1], ref_fields[i][1], tdoms[i]["solid"]) for i in idx]
     line = {"metric": "time-to-1e-6 residual vs plain FP32 GPU (learned warm start)",
             "value": warm["speedup_median"], "unit": "x", "higher_is_better": True,
-            "dtype": "f32 solver, bf16 surrogate", "data": "synthetic seeded obstacle channels",
+            "dtype": "f64 solver arithmetic on f32 storage, bf16 MFMA surrogate", "data": "synthetic seeded obstacle channels",
             "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training "
                                    f"({'+ mirrored, ' if args.mirror else ''}{args.epochs} epochs) / {args.ntest} test "
                                    f"channels, each test channel solved alone"},
@@ -504,7 +508,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.storage,
+        "dtype": COMPUTE[args.storage],
+        "storage": args.storage,
         "data": "synthetic (Taylor-Green vortex u0=0.05 + Gaussian noise sigma=1e-3, seed 1234)",
         "config": {"workload": f"D3Q19 BGK periodic box {n}^3, tau=0.6, {args.storage} populations (shifted f-w), "
                                f"fused pull collide+stream", "cells": cells, "parallelism": "single GPU",
