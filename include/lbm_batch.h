@@ -1,7 +1,8 @@
 /*
  * lbm_batch.h -- batches of many small, uneven lattice-Boltzmann domains in
- * one persistent launch per step, K steps plus the convergence check captured
- * in one hipGraph (part of liblbm_hip.so).
+ * one launch per step (one block per tile or a persistent grid, as the
+ * autotuner chooses), K steps plus the convergence check captured in one
+ * hipGraph (part of liblbm_hip.so).
  *
  * No reference counterpart: SFCMM/LBM runs one domain per process
  * (src/lbm/solver.cpp:176-214).  Each domain here is a structured box with its
