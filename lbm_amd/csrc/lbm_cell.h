@@ -725,11 +725,13 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
         // same source, or the cell's own entry when the source is missing)
         h[j] = pre[j];
       } else {
-        // a missing upstream cell (across a face) loads the cell's own entry
-        // of plane j (in bounds, same cache lines as the neighbours' loads)
-        // and is replaced below, so every load of the loop keeps the uniform
-        // base of plane j
-        const ST *p = pop_ptr<ST, W64>(src, plane, j, valid ? s : cell);
+        // a population whose upstream cell is missing (across a face, or a
+        // solid by the link bits) loads its bounce-back source S(x, opp j)
+        // at once, so the boundary rules below start from it without a
+        // second round of loads (the boundary waves of the 512x128x128
+        // sphere channel waited for that round)
+        const bool miss = !valid || (FLAGS && j < Q - 1 && ((lm >> j) & 1u));
+        const ST *p = pop_ptr<ST, W64>(src, plane, miss ? L::opp[j] : j, miss ? cell : s);
         h[j] = NT ? Store<ST>::ld_nt(p) : Store<ST>::ld(p);
       }
       if (!valid) missing |= 1u << j;
@@ -757,10 +759,12 @@ __device__ __forceinline__ void cell_update(const G &a, int x, int y, int z, dou
           ffkind = a.face[f].kind;
         }
       const bool wet = wet_kind(ffkind);
-      // bounce-back source S(x, opp j) of every missing population
+      // bounce-back source S(x, opp j) of every missing population (the
+      // gather loaded it already, except from the LDS stage)
+      if (LDSX)
 #pragma unroll
-      for (int j = 0; j < Q - 1; ++j)
-        if ((missing >> j) & 1u) h[j] = Store<ST>::ld(pop_ptr<ST, W64>(src, plane, L::opp[j], cell));
+        for (int j = 0; j < Q - 1; ++j)
+          if ((missing >> j) & 1u) h[j] = Store<ST>::ld(pop_ptr<ST, W64>(src, plane, L::opp[j], cell));
       // boundary substitution for populations whose upstream cell is missing.
       // First pass: the face each one crosses (the higher priority wins,
       // ties to the lower axis), walls applied at once; anti-bounce-back
