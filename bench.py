@@ -323,9 +323,11 @@ def warmstart_eval(args):
     line = {"metric": "time-to-1e-6 residual vs plain FP32 GPU (learned warm start)",
             "value": warm["speedup_median"], "unit": "x", "higher_is_better": True,
             "dtype": "f64 solver arithmetic on f32 storage, bf16 MFMA surrogate", "data": "synthetic seeded obstacle channels",
-            "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, {args.ntrain} training "
-                                   f"({'+ mirrored, ' if args.mirror else ''}{args.epochs} epochs) / {args.ntest} test "
-                                   f"channels, each test channel solved alone"},
+            "config": {"workload": f"D2Q9 256x128 obstacle channels, Re 20..200, "
+                                   + (f"network loaded from {os.path.basename(args.load_model)}" if args.load_model else
+                                      f"{args.ntrain} training ({'+ mirrored, ' if args.mirror else ''}"
+                                      f"{args.epochs} epochs)")
+                                   + f" / {args.ntest} test channels, each test channel solved alone"},
             "cold_f32": cold, "warm_f32": warm, "warm_f16_then_f32": dict(two, tol_f16=args.tol16),
             "converged_cases": int(ok.sum()), "prediction_u_rel_l2_median": float(np.median(e_pred)),
             "predict_s_per_case": round(pred_per_case, 4),
@@ -535,22 +537,30 @@ def main():
 
 
 def time_to_residual():
-    """The metric's second half, bounded to about a minute of GPU time:
+    """The metric's second half, bounded to about two minutes of GPU time:
     the 512x128x128 sphere channel to the relative residual 1e-6 in FP32
     (capped at 80,000 steps), and the learned warm start against the plain
-    FP32 start on 8 unseen 256x128 obstacle channels with a network trained
-    in this run on 1024 channels (+ mirrored, 30 epochs) by the project's
-    own kernels.  Skipped under rocprofv3 (the warm start's batched solves
-    do not run under it)."""
+    FP32 start on 8 unseen 256x128 obstacle channels, twice: with a network
+    trained in this run on 1024 channels (+ mirrored, 30 epochs) by the
+    project's own kernels, and with the network that ships with the package
+    (the same kernels, trained offline on 8192 channels + mirrored, 120
+    epochs).  Skipped under rocprofv3 (the warm start's batched solves do not
+    run under it)."""
     t0 = time.perf_counter()
     sph = sphere_solve("f32", 80000)
     sph.pop("u")
     ws = warmstart_eval(argparse.Namespace(load_model=None, ntrain=1024, ntest=8, epochs=30, lr=5e-4, mirror=1,
                                            max_steps=400000, tol16=1e-3, save_model=None))
     keep = ("cold_f32", "warm_f32", "converged_cases", "prediction_u_rel_l2_median", "predict_s_per_case", "training")
+    from lbm_amd import warmstart as WS
+    wp = warmstart_eval(argparse.Namespace(load_model=WS.SHIPPED_MODEL, ntrain=8192, ntest=8, epochs=120, lr=5e-4,
+                                           mirror=1, max_steps=400000, tol16=1e-3, save_model=None))
     return {"sphere_512x128x128_f32": sph,
             "warm_start": {"speedup_median_vs_plain_f32": ws["value"], "config": ws["config"]["workload"],
                            **{k: ws[k] for k in keep}},
+            "warm_start_shipped_network": {"speedup_median_vs_plain_f32": wp["value"],
+                                           "config": wp["config"]["workload"],
+                                           **{k: wp[k] for k in keep if k != "training"}},
             "wall_s": round(time.perf_counter() - t0, 1)}
 
 

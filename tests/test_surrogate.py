@@ -169,3 +169,25 @@ def test_gpu_gradient_deterministic_and_training_reduces_loss():
     y = s.predict_dataset(0, 4)
     assert np.allclose(y, s.forward(x[:4]))
     s.close()
+
+
+def test_shipped_warm_start_network_file():
+    """The trained network that ships with the package is stored as bf16
+    bit patterns (round to nearest even, as torch rounds float32 to
+    bfloat16) and holds exactly one value per parameter of the
+    architecture, plus the 3-channel target normalisation."""
+    import torch
+    from lbm_amd import warmstart as WS
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.standard_normal(4096).astype(np.float32) * 10.0 ** rng.integers(-6, 6, 4096),
+                        np.float32([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -1.5, 0.0, -0.0])])
+    bits = WS.bf16_bits(v)
+    want = torch.from_numpy(v).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    assert np.array_equal(bits, want)
+    assert np.array_equal((bits.astype(np.uint32) << 16).view(np.float32), SO.bf16(v).astype(np.float32))
+    z = np.load(WS.SHIPPED_MODEL)
+    assert sorted(z.files) == ["mu", "params_bf16", "sd"]
+    assert z["params_bf16"].dtype == np.uint16 and z["params_bf16"].size == SO.build()[3]
+    p = (z["params_bf16"].astype(np.uint32) << 16).view(np.float32)
+    assert np.isfinite(p).all() and 0.01 < float(np.abs(p).mean()) < 1.0
+    assert z["mu"].shape == (3,) and z["sd"].shape == (3,) and (z["sd"] > 0).all()
