@@ -543,7 +543,7 @@ def time_to_residual():
     FP32 start on 8 unseen 256x128 obstacle channels, twice: with a network
     trained in this run on 1024 channels (+ mirrored, 30 epochs) by the
     project's own kernels, and with the network that ships with the package
-    (the same kernels, trained offline on 8192 channels + mirrored, 120
+    (the same kernels, trained offline on 8192 channels + mirrored, 240
     epochs).  Skipped under rocprofv3 (the warm start's batched solves do not
     run under it)."""
     t0 = time.perf_counter()
@@ -553,7 +553,7 @@ def time_to_residual():
                                            max_steps=400000, tol16=1e-3, save_model=None))
     keep = ("cold_f32", "warm_f32", "converged_cases", "prediction_u_rel_l2_median", "predict_s_per_case", "training")
     from lbm_amd import warmstart as WS
-    wp = warmstart_eval(argparse.Namespace(load_model=WS.SHIPPED_MODEL, ntrain=8192, ntest=8, epochs=120, lr=5e-4,
+    wp = warmstart_eval(argparse.Namespace(load_model=WS.SHIPPED_MODEL, ntrain=8192, ntest=8, epochs=240, lr=5e-4,
                                            mirror=1, max_steps=400000, tol16=1e-3, save_model=None))
     return {"sphere_512x128x128_f32": sph,
             "warm_start": {"speedup_median_vs_plain_f32": ws["value"], "config": ws["config"]["workload"],

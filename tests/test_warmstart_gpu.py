@@ -91,7 +91,7 @@ MODEL = WS.SHIPPED_MODEL
 @pytest.mark.timeout(600)
 def test_shipped_network_saves_steps_on_unseen_channels():
     """The network that ships with the package (trained with these kernels
-    on 8192 channels and their mirror images for 120 epochs), on 6 unseen
+    on 8192 channels and their mirror images for 240 epochs), on 6 unseen
     channels of another seed: the median step ratio (plain / warm) is above
     1.5 (the small network trained inside the test above reaches ~1.6 on
     its own channels), no warm start costs more than 1.3x the plain start's
