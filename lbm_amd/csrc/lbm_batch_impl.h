@@ -444,7 +444,9 @@ int lbm_batch_create(int32_t lattice, int32_t storage, const lbm_domain_desc *do
     D.nx = dd.nx;
     D.ny = dd.ny;
     D.nz = dd.nz;
-    D.px = (dd.nx + 31) / 32 * 32;  // rows start on 128-byte lines (FP32)
+    // rows start on 128-byte lines: 16 (FP64), 32 (FP32) or 64 (16-bit) cells
+    const int line = 128 / es;
+    D.px = (dd.nx + line - 1) / line * line;
     D.plane = (long long)D.px * D.ny * D.nz;
     if ((unsigned long long)D.plane * (unsigned long long)es > 0xffffffffull) {
       bfail(nullptr, LBM_EINVAL, "lbm_batch_create: domain " + std::to_string(d) +
